@@ -1,0 +1,134 @@
+/*
+ * katric_amd -- C ABI of the MI355X (gfx950) triangle-counting core.
+ *
+ * The reference (KaTric, header-only C++) has no C ABI; its drop-in surface
+ * for this path is a set of C++ templates.  Each entry point below names the
+ * reference interface it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *   - vertex ids uint32, CSR offsets uint64 (n+1 entries, offsets[0] = 0),
+ *     triangle counts uint64; all counting is exact integer arithmetic.
+ *   - input graphs are symmetric (undirected) CSRs without self-loops or
+ *     duplicate entries; adjacency lists may be in any order.
+ *   - pointers are host pointers unless KT_DEVICE_INPUT / KT_DEVICE_OUTPUT
+ *     say otherwise; device pointers must be allocations on the context's
+ *     device whose producing work has completed before the call.
+ *   - every call is synchronous: outputs are complete when it returns.
+ *   - status codes are 0 (KT_OK) or one of the negative KT_E* values; no
+ *     C++ exception crosses the ABI.  kt_last_error() gives a message.
+ *   - one kt_ctx may be used from several host threads; calls on one
+ *     context are serialised by an internal lock.
+ */
+#ifndef KATRIC_H
+#define KATRIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KATRIC_ABI_VERSION 1
+
+/* status codes */
+#define KT_OK 0
+#define KT_EINVAL (-1)   /* malformed CSR, ranges or arguments */
+#define KT_ENOMEM (-2)   /* HBM budget or device allocation exhausted */
+#define KT_EHIP (-3)     /* HIP runtime error */
+#define KT_ENODEV (-4)   /* no usable GPU / bad device index */
+#define KT_ENOTREADY (-5) /* serve: request still pending */
+#define KT_ECANCELED (-6) /* serve: request cancelled */
+#define KT_ENOTFOUND (-7) /* serve: unknown ticket */
+#define KT_ESTATE (-8)    /* call not valid in the current state */
+
+/* flags */
+#define KT_DEVICE_INPUT 0x1u  /* offsets / heads are device pointers */
+#define KT_DEVICE_OUTPUT 0x2u /* per-vertex outputs are device pointers */
+
+typedef struct kt_ctx kt_ctx;
+
+/* Counters of cetric::profiling::LocalStatistics for the last call
+ * (include/cetric/statistics.h:193-203).  intersection_size_local is the
+ * reference's W = sum over oriented edges (v,u) of |N+(v)| + |N+(u)|
+ * (counters/cetric_edge_iterator.h:279-281). */
+typedef struct {
+    uint64_t wedge_checks;
+    uint64_t intersection_size_local;
+    uint64_t intersection_size_global;
+    uint64_t local_triangles;
+    uint64_t type3_triangles;
+    uint64_t oriented_edges; /* |E+| */
+    uint64_t vertices;
+    uint64_t input_slots;    /* entries of the symmetric input CSR */
+} kt_stats;
+
+/* Device times of the last call, from HIP events on the context stream. */
+typedef struct {
+    float total_ms;     /* first device op to last device op */
+    float h2d_ms;       /* host -> device copies (0 for device input) */
+    float prep_ms;      /* orientation + oriented-CSR construction */
+    float count_ms;     /* binning + intersection */
+    float intersect_ms; /* the intersection kernel alone */
+    float d2h_ms;       /* device -> host copies of the outputs */
+} kt_timing;
+
+/* Intersection strategy knobs (bin thresholds); see kt_tune_* in the
+ * package for the autotuner that chooses them. */
+typedef struct {
+    uint32_t wave_max_deg;  /* |N+(v)| <= this: one wave per tail, LDS hash (<= 512) */
+    uint32_t group_max_deg; /* |N+(v)| <= this: one workgroup per tail, LDS hash (<= 2048) */
+    uint32_t grid_per_cu;   /* persistent workgroups per compute unit */
+    uint32_t reserved;
+} kt_params;
+
+/* Create a context on HIP device `device`.  hbm_budget_bytes bounds the
+ * device memory the context may hold (0 = no bound beyond the device).
+ * No reference counterpart (the reference allocates on the host). */
+int kt_create(kt_ctx** ctx, int device, uint64_t hbm_budget_bytes);
+void kt_destroy(kt_ctx* ctx);
+const char* kt_strerror(int status);
+const char* kt_last_error(const kt_ctx* ctx);
+int kt_set_params(kt_ctx* ctx, const kt_params* params);
+int kt_get_params(const kt_ctx* ctx, kt_params* params);
+int kt_last_stats(const kt_ctx* ctx, kt_stats* stats);
+int kt_last_timing(const kt_ctx* ctx, kt_timing* timing);
+/* bytes of device memory currently held by the context */
+uint64_t kt_device_bytes(const kt_ctx* ctx);
+
+/* Device buffers on the context's device, for callers without a HIP
+ * allocator of their own (they count against the HBM budget).  No reference
+ * counterpart: the reference keeps every graph in host memory. */
+int kt_buffer_alloc(kt_ctx* ctx, uint64_t bytes, void** device_ptr);
+int kt_buffer_free(kt_ctx* ctx, void* device_ptr);
+int kt_buffer_upload(kt_ctx* ctx, void* device_dst, const void* host_src, uint64_t bytes);
+int kt_buffer_download(kt_ctx* ctx, void* host_dst, const void* device_src, uint64_t bytes);
+/* Waits for all work of the context's device. */
+int kt_synchronize(kt_ctx* ctx);
+
+/* Shared-memory triangle count of a symmetric CSR: total and (if
+ * per_vertex_out != NULL) per-vertex counts t[x] = number of triangles
+ * containing x.  Replaces cetric::run_shmem
+ * (include/cetric/counters/shared_memory_cetric.h:105-170), i.e.
+ * preprocessing (:71-103: DistributedGraph::orient with
+ * node_ordering::degree_outward + sort_neighborhoods,
+ * datastructures/distributed/distributed_graph.h:537-593) followed by
+ * CetricEdgeIterator::run_local (counters/cetric_edge_iterator.h:173-298,
+ * 888-1066) whose emit callback counts triangles. */
+int kt_count(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+             uint64_t* total_out, uint64_t* per_vertex_out);
+
+/* Degree orientation + oriented-CSR construction only: out_offsets[n+1],
+ * out_heads[offsets[n]/2] receive, for every v, the ascending ids u with
+ * (deg v, v) < (deg u, u).  Replaces the preprocessing half of run_shmem:
+ * DistributedGraph::orient (distributed_graph.h:537-566) with
+ * node_ordering::degree_outward (:147-163), sort_neighborhoods (:577-593)
+ * and the out-only rewrite of remove_in_edges_and_expand_ghosts
+ * (:1057-1104) on one rank. */
+int kt_orient(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+              uint64_t* out_offsets, uint32_t* out_heads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KATRIC_H */

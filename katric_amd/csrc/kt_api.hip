@@ -1,0 +1,557 @@
+// C ABI of katric_amd: context, device workspace, plain count / orient.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "../../include/katric.h"
+#include "kt_ctx.h"
+
+using namespace kt;
+
+bool kt::trace_enabled() {
+    static int on = -1;
+    if (on < 0) {
+        const char* e = getenv("KATRIC_TRACE");
+        on = (e && e[0] && e[0] != '0') ? 1 : 0;
+    }
+    return on == 1;
+}
+
+void kt::trace(const char* fmt, ...) {
+    if (!trace_enabled()) return;
+    va_list ap;
+    va_start(ap, fmt);
+    fprintf(stderr, "[katric] ");
+    vfprintf(stderr, fmt, ap);
+    fputc('\n', stderr);
+    fflush(stderr);
+    va_end(ap);
+}
+
+// ---------------------------------------------------------------------------
+// context helpers
+// ---------------------------------------------------------------------------
+int kt_ctx::fail(int status, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, sizeof(err), fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+int kt_ctx::hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return KT_OK;
+    (void)hipGetLastError();
+    return fail(KT_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int kt_ctx::reserve(uint64_t bytes) {
+    bytes = (bytes + 255) & ~255ull;
+    if (bytes <= arena_bytes) return KT_OK;
+    uint64_t want = std::max<uint64_t>(bytes, arena_bytes + arena_bytes / 4);
+    if (budget && pinned_other + want > budget) want = bytes;
+    if (budget && pinned_other + want > budget)
+        return fail(KT_ENOMEM, "workspace of %llu bytes exceeds the HBM budget of %llu bytes",
+                    (unsigned long long)want, (unsigned long long)budget);
+    if (arena) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_check(e, "hipStreamSynchronize");
+        (void)hipFree(arena);
+        arena = nullptr;
+        arena_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&arena, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        arena = nullptr;
+        return fail(KT_ENOMEM, "hipMalloc(%llu) failed: %s", (unsigned long long)want, hipGetErrorString(e));
+    }
+    arena_bytes = want;
+    return KT_OK;
+}
+
+void* Carver::take(uint64_t bytes) {
+    uint64_t at = (used + 255) & ~255ull;
+    used = at + ((bytes + 255) & ~255ull);
+    return base ? static_cast<char*>(base) + at : nullptr;
+}
+
+int kt_ctx::begin_call(bool counting) {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_check(e, "hipSetDevice");
+    err[0] = 0;
+    if (counting) {
+        memset(&stats, 0, sizeof(stats));
+        memset(&timing, 0, sizeof(timing));
+    }
+    return KT_OK;
+}
+
+float kt_ctx::elapsed(int a, int b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+// ---------------------------------------------------------------------------
+// lifecycle
+// ---------------------------------------------------------------------------
+extern "C" int kt_create(kt_ctx** out, int device, uint64_t hbm_budget_bytes) {
+    if (!out) return KT_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return KT_ENODEV;
+    }
+    if (device < 0 || device >= count) return KT_ENODEV;
+    if (hipSetDevice(device) != hipSuccess) {
+        (void)hipGetLastError();
+        return KT_ENODEV;
+    }
+    kt_ctx* c = new (std::nothrow) kt_ctx();
+    if (!c) return KT_ENOMEM;
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return KT_EHIP;
+    }
+    c->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return KT_EHIP;
+    }
+    for (int i = 0; i < KT_NUM_EVENTS; ++i) {
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            (void)hipGetLastError();
+            kt_destroy(c);
+            return KT_EHIP;
+        }
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->host_small), 64 * sizeof(uint64_t), hipHostMallocDefault) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        c->host_small = nullptr;
+        kt_destroy(c);
+        return KT_EHIP;
+    }
+    c->prm.wave_max_deg = 512;
+    c->prm.group_max_deg = 2048;
+    c->prm.split_chunks = 0;
+    c->prm.grid_per_cu = 4;
+    trace("context on device %d: %d CUs, stream %p", device, c->num_cus, (void*)c->stream);
+    *out = c;
+    return KT_OK;
+}
+
+extern "C" void kt_destroy(kt_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    c->destroy_extensions();
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->arena) (void)hipFree(c->arena);
+    if (c->host_small) (void)hipHostFree(c->host_small);
+    for (int i = 0; i < KT_NUM_EVENTS; ++i)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    (void)hipGetLastError();
+    delete c;
+}
+
+extern "C" const char* kt_strerror(int s) {
+    switch (s) {
+        case KT_OK: return "ok";
+        case KT_EINVAL: return "invalid argument or malformed graph";
+        case KT_ENOMEM: return "device memory budget exhausted";
+        case KT_EHIP: return "HIP runtime error";
+        case KT_ENODEV: return "no usable GPU device";
+        case KT_ENOTREADY: return "request not finished";
+        case KT_ECANCELED: return "request cancelled";
+        case KT_ENOTFOUND: return "unknown request";
+        case KT_ESTATE: return "call not valid in the current state";
+        default: return "unknown status";
+    }
+}
+
+extern "C" const char* kt_last_error(const kt_ctx* c) { return c ? c->err : "no context"; }
+
+extern "C" int kt_set_params(kt_ctx* c, const kt_params* p) {
+    if (!c || !p) return KT_EINVAL;
+    if (p->wave_max_deg < 2 || p->wave_max_deg > 512 || p->group_max_deg < p->wave_max_deg ||
+        p->group_max_deg > 2048 || p->grid_per_cu < 1 || p->grid_per_cu > 32)
+        return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->prm.wave_max_deg = p->wave_max_deg;
+    c->prm.group_max_deg = p->group_max_deg;
+    c->prm.grid_per_cu = p->grid_per_cu;
+    return KT_OK;
+}
+
+extern "C" int kt_get_params(const kt_ctx* c, kt_params* p) {
+    if (!c || !p) return KT_EINVAL;
+    p->wave_max_deg = c->prm.wave_max_deg;
+    p->group_max_deg = c->prm.group_max_deg;
+    p->grid_per_cu = c->prm.grid_per_cu;
+    p->reserved = 0;
+    return KT_OK;
+}
+
+extern "C" int kt_last_stats(const kt_ctx* c, kt_stats* s) {
+    if (!c || !s) return KT_EINVAL;
+    *s = c->stats;
+    return KT_OK;
+}
+
+extern "C" int kt_last_timing(const kt_ctx* c, kt_timing* t) {
+    if (!c || !t) return KT_EINVAL;
+    *t = c->timing;
+    return KT_OK;
+}
+
+extern "C" uint64_t kt_device_bytes(const kt_ctx* c) { return c ? c->arena_bytes + c->pinned_other : 0; }
+
+// ---------------------------------------------------------------------------
+// plain path: orientation + intersection
+// ---------------------------------------------------------------------------
+namespace {
+
+struct PlainBufs {
+    const uint64_t* off;
+    const uint32_t* heads;
+    uint32_t* deg;
+    uint64_t* ooff;
+    uint32_t* oheads;
+    uint64_t* keys;
+    uint64_t* alt;
+    uint64_t* ws_orient;
+    uint64_t* ws_tc;
+    uint64_t* ws_radix;
+    uint64_t* pv;
+    uint64_t* small;  // [0] = err flags (u32), [1] = total, [2..] = tc stats
+};
+
+// Lays out every buffer of the plain path; with base == nullptr only sizes.
+uint64_t plain_layout(void* base, uint32_t n, uint64_t nnz, bool host_in, bool want_pv_dev, bool unsorted,
+                      PlainBufs& B) {
+    Carver cv{base, 0};
+    const uint64_t m = nnz / 2;
+    B.small = static_cast<uint64_t*>(cv.take(64 * sizeof(uint64_t)));
+    B.off = nullptr;
+    B.heads = nullptr;
+    if (host_in) {
+        B.off = static_cast<uint64_t*>(cv.take((uint64_t(n) + 1) * 8));
+        B.heads = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(nnz, 1) * 4));
+    }
+    B.deg = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(n, 1) * 4));
+    B.ooff = static_cast<uint64_t*>(cv.take((uint64_t(n) + 1) * 8));
+    B.oheads = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(m, 1) * 4));
+    B.ws_orient = static_cast<uint64_t*>(cv.take(orient_ws_words(n, nnz) * 8));
+    B.ws_tc = static_cast<uint64_t*>(cv.take(tc_ws_words(n) * 8));
+    B.pv = want_pv_dev ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(n, 1) * 8)) : nullptr;
+    B.keys = B.alt = B.ws_radix = nullptr;
+    if (unsorted) {
+        B.keys = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(m, 1) * 8));
+        B.alt = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(m, 1) * 8));
+        B.ws_radix = static_cast<uint64_t*>(cv.take(radix_tmp_words(m) * 8));
+    }
+    return cv.used;
+}
+
+int bits_for(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b)) ++b;
+    return b;
+}
+
+// Uploads host input (if any) and runs the count pass of the orientation.
+int upload_and_count(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_off, const uint32_t* user_heads,
+                     bool host_in, PlainBufs& B) {
+    hipStream_t st = c->stream;
+    int rc;
+    if (host_in) {
+        rc = c->hip_check(hipMemcpyAsync(const_cast<uint64_t*>(B.off), user_off, (uint64_t(n) + 1) * 8,
+                                         hipMemcpyHostToDevice, st),
+                          "H2D offsets");
+        if (rc) return rc;
+        if (nnz) {
+            rc = c->hip_check(hipMemcpyAsync(const_cast<uint32_t*>(B.heads), user_heads, nnz * 4,
+                                             hipMemcpyHostToDevice, st),
+                              "H2D heads");
+            if (rc) return rc;
+        }
+    } else {
+        B.off = user_off;
+        B.heads = user_heads;
+    }
+    hipEventRecord(c->ev[EV_H2D], st);
+    trace("upload done, launching count pass");
+    fill_u64(B.small, 64, 0, st);
+    orient_count_launch(n, nnz, B.off, B.heads, B.deg, B.ws_orient, reinterpret_cast<uint32_t*>(B.small), st);
+    rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st), "D2H flags");
+    if (rc) return rc;
+    rc = c->hip_check(
+        hipMemcpyAsync(c->host_small + 1, orient_total_ptr(nnz, B.ws_orient), 8, hipMemcpyDeviceToHost, st),
+        "D2H kept");
+    if (rc) return rc;
+    return c->hip_check(hipStreamSynchronize(st), "orientation count");
+}
+
+// Builds the oriented CSR in B.ooff / B.oheads.  The count pass is checked on
+// the host (one stream sync) before anything is written, so malformed or
+// asymmetric input never reaches the write pass or the intersection.
+int run_orientation(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_off, const uint32_t* user_heads,
+                    bool host_in, bool pv_dev_needed, PlainBufs& B, bool& unsorted) {
+    hipStream_t st = c->stream;
+    unsorted = false;
+    uint64_t need = plain_layout(nullptr, n, nnz, host_in, pv_dev_needed, false, B);
+    trace("orientation: n=%u nnz=%llu workspace=%llu", n, (unsigned long long)nnz, (unsigned long long)need);
+    int rc = c->reserve(need);
+    if (rc) return rc;
+    plain_layout(c->arena, n, nnz, host_in, pv_dev_needed, false, B);
+    trace("arena=%p bytes=%llu", c->arena, (unsigned long long)c->arena_bytes);
+    hipEventRecord(c->ev[EV_START], st);
+    rc = upload_and_count(c, n, nnz, user_off, user_heads, host_in, B);
+    if (rc) return rc;
+    uint32_t flags = static_cast<uint32_t>(c->host_small[0]);
+    uint64_t kept = c->host_small[1];
+    trace("count pass: flags=0x%x kept=%llu", flags, (unsigned long long)kept);
+    if (flags & KT_ERR_MASK) {
+        return c->fail(KT_EINVAL, "malformed CSR:%s%s%s%s", (flags & KT_ERR_OFFSETS) ? " bad offsets" : "",
+                       (flags & KT_ERR_HEAD_RANGE) ? " head id >= n" : "",
+                       (flags & KT_ERR_SELF_LOOP) ? " self-loop" : "",
+                       (flags & KT_ERR_DUPLICATE) ? " duplicate edge" : "");
+    }
+    if (2 * kept != nnz) {
+        return c->fail(KT_EINVAL, "CSR is not symmetric (%llu oriented edges for %llu slots)",
+                       (unsigned long long)kept, (unsigned long long)nnz);
+    }
+    const uint64_t m = nnz / 2;
+    if (flags & KT_FLAG_UNSORTED) {
+        unsorted = true;
+        need = plain_layout(nullptr, n, nnz, host_in, pv_dev_needed, true, B);
+        if (need > c->arena_bytes) {
+            // growing the arena drops its contents: upload and count again
+            rc = c->reserve(need);
+            if (rc) return rc;
+            plain_layout(c->arena, n, nnz, host_in, pv_dev_needed, true, B);
+            rc = upload_and_count(c, n, nnz, user_off, user_heads, host_in, B);
+            if (rc) return rc;
+        } else {
+            plain_layout(c->arena, n, nnz, host_in, pv_dev_needed, true, B);
+            if (!host_in) {
+                B.off = user_off;
+                B.heads = user_heads;
+            }
+        }
+        orient_write_launch(n, nnz, B.off, B.heads, B.deg, B.ooff, nullptr, B.keys, B.ws_orient, st);
+        radix_sort_u64(B.keys, B.alt, m, 32 + bits_for(n), B.ws_radix, st);
+        keys_to_heads(m, B.keys, B.oheads, reinterpret_cast<uint32_t*>(B.small), st);
+        rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st), "D2H flags");
+        if (rc) return rc;
+        rc = c->hip_check(hipStreamSynchronize(st), "sort");
+        if (rc) return rc;
+        flags = static_cast<uint32_t>(c->host_small[0]);
+        if (flags & KT_ERR_MASK) return c->fail(KT_EINVAL, "malformed CSR: duplicate edge");
+    } else {
+        orient_write_launch(n, nnz, B.off, B.heads, B.deg, B.ooff, B.oheads, nullptr, B.ws_orient, st);
+    }
+    c->stats.oriented_edges = kept;
+    c->stats.vertices = n;
+    c->stats.input_slots = nnz;
+    return KT_OK;
+}
+
+int read_nnz(kt_ctx* c, uint32_t n, const uint64_t* offsets, uint32_t flags, uint64_t& nnz) {
+    if (flags & KT_DEVICE_INPUT) {
+        uint64_t v[2] = {0, 0};
+        int rc = c->hip_check(hipMemcpy(v, offsets, 8, hipMemcpyDeviceToHost), "D2H offsets[0]");
+        if (rc) return rc;
+        rc = c->hip_check(hipMemcpy(v + 1, offsets + n, 8, hipMemcpyDeviceToHost), "D2H offsets[n]");
+        if (rc) return rc;
+        if (v[0] != 0) return c->fail(KT_EINVAL, "offsets[0] != 0");
+        nnz = v[1];
+    } else {
+        if (offsets[0] != 0) return c->fail(KT_EINVAL, "offsets[0] != 0");
+        nnz = offsets[n];
+    }
+    if (nnz & 1) return c->fail(KT_EINVAL, "odd number of CSR entries: not a symmetric graph");
+    if (nnz / 2 > 0xffffffffull * 2) return c->fail(KT_EINVAL, "graph too large");
+    return KT_OK;
+}
+
+}  // namespace
+
+extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+                        uint64_t* total_out, uint64_t* per_vertex_out) {
+    if (!c) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(true);
+    if (rc) return rc;
+    if (!total_out || !offsets || (n > 0 && !heads && !(flags & KT_DEVICE_INPUT)))
+        return c->fail(KT_EINVAL, "null argument");
+    *total_out = 0;
+    if (n == 0) return KT_OK;
+    uint64_t nnz = 0;
+    rc = read_nnz(c, n, offsets, flags, nnz);
+    if (rc) return rc;
+    const bool host_in = !(flags & KT_DEVICE_INPUT);
+    const bool want_pv = per_vertex_out != nullptr;
+    const bool pv_dev = want_pv && (flags & KT_DEVICE_OUTPUT);
+    PlainBufs B;
+    bool unsorted;
+    rc = run_orientation(c, n, nnz, offsets, heads, host_in, want_pv && !pv_dev, B, unsorted);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    hipEventRecord(c->ev[EV_PREP], st);
+    uint64_t* pv = want_pv ? (pv_dev ? per_vertex_out : B.pv) : nullptr;
+    if (pv) fill_u64(pv, n, 0, st);
+    uint64_t* totals = B.small + 1;
+    TcView view{};
+    view.X = n;
+    view.s_off = B.ooff;
+    view.sheads = B.oheads;
+    TcOut out{pv, totals};
+    trace("intersection: X=%u pv=%p", n, (void*)pv);
+    tc_launch(view, out, c->prm, B.ws_tc, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]);
+    hipEventRecord(c->ev[EV_COUNT], st);
+    uint64_t* tstats = tc_stats_ptr(B.ws_tc);
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 1, totals, 8, hipMemcpyDeviceToHost, st), "D2H total");
+    if (rc) return rc;
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
+    if (rc) return rc;
+    if (want_pv && !pv_dev) {
+        rc = c->hip_check(hipMemcpyAsync(per_vertex_out, pv, uint64_t(n) * 8, hipMemcpyDeviceToHost, st),
+                          "D2H per-vertex");
+        if (rc) return rc;
+    }
+    hipEventRecord(c->ev[EV_END], st);
+    rc = c->hip_check(hipStreamSynchronize(st), "count");
+    if (rc) return rc;
+    trace("count done: total=%llu", (unsigned long long)c->host_small[1]);
+    *total_out = c->host_small[1];
+    c->stats.local_triangles = c->host_small[1];
+    c->stats.wedge_checks = c->host_small[2];
+    c->stats.intersection_size_local = c->host_small[3];
+    c->timing.h2d_ms = c->elapsed(EV_START, EV_H2D);
+    c->timing.prep_ms = unsorted ? 0.f : c->elapsed(EV_H2D, EV_PREP);
+    c->timing.count_ms = c->elapsed(EV_PREP, EV_COUNT);
+    c->timing.intersect_ms = c->elapsed(EV_TC0, EV_TC1);
+    c->timing.d2h_ms = c->elapsed(EV_COUNT, EV_END);
+    c->timing.total_ms = unsorted ? 0.f : c->elapsed(EV_START, EV_END);
+    return KT_OK;
+}
+
+extern "C" int kt_orient(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+                         uint64_t* out_offsets, uint32_t* out_heads) {
+    if (!c) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(true);
+    if (rc) return rc;
+    if (!offsets || !out_offsets) return c->fail(KT_EINVAL, "null argument");
+    if (n == 0) {
+        if (flags & KT_DEVICE_OUTPUT)
+            return c->hip_check(hipMemset(out_offsets, 0, 8), "memset");
+        out_offsets[0] = 0;
+        return KT_OK;
+    }
+    uint64_t nnz = 0;
+    rc = read_nnz(c, n, offsets, flags, nnz);
+    if (rc) return rc;
+    PlainBufs B;
+    bool unsorted;
+    rc = run_orientation(c, n, nnz, offsets, heads, !(flags & KT_DEVICE_INPUT), false, B, unsorted);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    hipMemcpyKind kind = (flags & KT_DEVICE_OUTPUT) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    rc = c->hip_check(hipMemcpyAsync(out_offsets, B.ooff, (uint64_t(n) + 1) * 8, kind, st), "copy offsets");
+    if (rc) return rc;
+    if (nnz && out_heads) {
+        rc = c->hip_check(hipMemcpyAsync(out_heads, B.oheads, nnz / 2 * 4, kind, st), "copy heads");
+        if (rc) return rc;
+    }
+    return c->hip_check(hipStreamSynchronize(st), "orient");
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+extern "C" int kt_buffer_alloc(kt_ctx* c, uint64_t bytes, void** dptr) {
+    if (!c || !dptr) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(false);
+    if (rc) return rc;
+    *dptr = nullptr;
+    bytes = std::max<uint64_t>(bytes, 1);
+    if (c->budget && c->arena_bytes + c->pinned_other + bytes > c->budget)
+        return c->fail(KT_ENOMEM, "buffer of %llu bytes exceeds the HBM budget", (unsigned long long)bytes);
+    hipError_t e = hipMalloc(dptr, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *dptr = nullptr;
+        return c->fail(KT_ENOMEM, "hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
+    }
+    c->buffers[*dptr] = bytes;
+    c->pinned_other += bytes;
+    return KT_OK;
+}
+
+extern "C" int kt_buffer_free(kt_ctx* c, void* dptr) {
+    if (!c) return KT_EINVAL;
+    if (!dptr) return KT_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(false);
+    if (rc) return rc;
+    auto it = c->buffers.find(dptr);
+    if (it == c->buffers.end()) return c->fail(KT_EINVAL, "pointer was not allocated by kt_buffer_alloc");
+    rc = c->hip_check(hipStreamSynchronize(c->stream), "sync");
+    if (rc) return rc;
+    c->pinned_other -= it->second;
+    c->buffers.erase(it);
+    return c->hip_check(hipFree(dptr), "hipFree");
+}
+
+extern "C" int kt_buffer_upload(kt_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(false);
+    if (rc) return rc;
+    if (!bytes) return KT_OK;
+    rc = c->hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+    if (rc) return rc;
+    return c->hip_check(hipStreamSynchronize(c->stream), "upload sync");
+}
+
+extern "C" int kt_buffer_download(kt_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(false);
+    if (rc) return rc;
+    if (!bytes) return KT_OK;
+    rc = c->hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "download");
+    if (rc) return rc;
+    return c->hip_check(hipStreamSynchronize(c->stream), "download sync");
+}
+
+extern "C" int kt_synchronize(kt_ctx* c) {
+    if (!c) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(false);
+    if (rc) return rc;
+    return c->hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+void kt_ctx::destroy_extensions() {
+    for (auto& kv : buffers) (void)hipFree(kv.first);
+    buffers.clear();
+}

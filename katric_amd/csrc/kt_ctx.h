@@ -1,0 +1,69 @@
+// kt_ctx: one device, one stream, one grow-only device arena.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <unordered_map>
+
+#include "../../include/katric.h"
+#include "kt_internal.h"
+
+enum {
+    EV_START = 0,
+    EV_H2D,
+    EV_PREP,
+    EV_TC0,
+    EV_TC1,
+    EV_COUNT,
+    EV_END,
+    EV_X0,
+    EV_X1,
+    EV_X2,
+    KT_NUM_EVENTS
+};
+
+namespace kt {
+// Bump allocator over the arena: with base == nullptr it only measures.
+struct Carver {
+    void* base;
+    uint64_t used;
+    void* take(uint64_t bytes);
+};
+struct ServeState;
+struct BatchState;
+}  // namespace kt
+
+struct kt_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    void* arena = nullptr;
+    uint64_t arena_bytes = 0;
+    uint64_t pinned_other = 0;  // device bytes held outside the arena (batch/serve pools)
+    hipEvent_t ev[KT_NUM_EVENTS] = {};
+    uint64_t* host_small = nullptr;  // pinned staging for flags / totals / stats
+    kt::TcParams prm{};
+    kt_stats stats{};
+    kt_timing timing{};
+    std::mutex mu;
+    char err[512] = {0};
+    kt::ServeState* serve = nullptr;
+    kt::BatchState* batch = nullptr;
+    std::unordered_map<void*, uint64_t> buffers;  // kt_buffer_alloc allocations
+
+    int fail(int status, const char* fmt, ...);
+    int hip_check(hipError_t e, const char* what);
+    int reserve(uint64_t bytes);
+    int begin_call(bool counting);
+    float elapsed(int a, int b);
+    void destroy_extensions();
+};
+
+namespace kt {
+uint64_t* tc_stats_ptr(uint64_t* ws);
+// KATRIC_TRACE=1: one stderr line per host-side pipeline step
+bool trace_enabled();
+void trace(const char* fmt, ...);
+}

@@ -1,0 +1,78 @@
+// Device-side helpers shared by the katric_amd HIP kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KT_WAVE 64
+#define KT_EMPTY 0xffffffffu
+
+namespace kt {
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+// Make this wave's LDS writes visible to its other lanes before they read.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    unsigned l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Inclusive prefix sum across the wave (Hillis-Steele over 6 steps).
+template <typename T>
+__device__ __forceinline__ T wave_incl_sum(T x) {
+    unsigned l = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T y = __shfl_up(x, d, 64);
+        if (l >= (unsigned)d) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int wave_incl_max(int x) {
+    unsigned l = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int y = __shfl_up(x, d, 64);
+        if (l >= (unsigned)d) x = max(x, y);
+    }
+    return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+__device__ __forceinline__ unsigned long long atomic_add_u64(uint64_t* p, uint64_t v) {
+    return atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
+}
+
+// Largest v in [lo, hi] with off[v] <= s (the CSR row holding slot s);
+// requires off[lo] <= s.
+__device__ __forceinline__ uint32_t row_of_slot(const uint64_t* __restrict__ off, uint32_t lo, uint32_t hi,
+                                                uint64_t s) {
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (off[mid] <= s)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Multiplicative (Fibonacci) hash into a 2^logS table.
+__device__ __forceinline__ uint32_t hash_slot(uint32_t w, uint32_t logS) {
+    return (w * 2654435761u) >> (32 - logS);
+}
+
+}  // namespace kt

@@ -1,0 +1,74 @@
+// Internal declarations shared by the katric_amd translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Device-side validation flags (bitmask accumulated with atomicOr).
+#define KT_ERR_OFFSETS 0x1u      // offsets not monotone / off[0] != 0 / off[n] != nnz
+#define KT_ERR_HEAD_RANGE 0x2u   // head id >= n
+#define KT_ERR_SELF_LOOP 0x4u    // head == tail
+#define KT_ERR_DUPLICATE 0x8u    // repeated head in one list
+#define KT_FLAG_UNSORTED 0x100u  // some list not ascending (not an error)
+#define KT_ERR_GHOST 0x10u       // shard: remote head missing from the ghost table
+#define KT_ERR_MESSAGE 0x20u     // shard: malformed message list
+#define KT_ERR_MASK 0xffu
+
+namespace kt {
+
+// scans / sorts / orientation (kt_prep.hip)
+uint64_t scan_tmp_words(uint64_t len);
+void scan_u64(const uint64_t* in, uint64_t* out, uint64_t len, uint64_t* tmp, hipStream_t st);
+uint64_t radix_tmp_words(uint64_t len);
+void radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t len, int bits, uint64_t* tmp, hipStream_t st);
+uint64_t orient_ws_words(uint32_t n, uint64_t nnz);
+void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,
+                         uint64_t* ws, uint32_t* err, hipStream_t st);
+const uint64_t* orient_total_ptr(uint64_t nnz, const uint64_t* ws);
+void orient_write_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, const uint32_t* deg,
+                         uint64_t* ooff, uint32_t* oheads, uint64_t* keys, const uint64_t* ws, hipStream_t st);
+void keys_to_heads(uint64_t m, const uint64_t* keys, uint32_t* oheads, uint32_t* err, hipStream_t st);
+
+// fill kernels (kt_util.hip); used instead of hipMemsetAsync so that every
+// pipeline stays capturable in a HIP graph.
+void fill_u64(uint64_t* p, uint64_t count, uint64_t value, hipStream_t st);
+void fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t st);
+
+// Intersection tuning knobs (bin thresholds, table sizes); see kt_tc.hip.
+struct TcParams {
+    uint32_t wave_max_deg;    // d+(v) <= this: one wave per tail vertex (LDS hash)
+    uint32_t group_max_deg;   // d+(v) <= this: one workgroup (4 waves) per tail vertex (LDS hash)
+    uint32_t split_chunks;    // >0: tails with more than this many 64-neighbour chunks are split
+    uint32_t grid_per_cu;     // persistent workgroups per CU
+};
+
+// Generic "tail list" view for the intersection kernel:
+//   tails x in [0, X): hash set S(x) = sheads[s_off[x] .. s_off[x+1]),
+//   iterated neighbours R(x) = rheads[r_beg[x] .. r_end[x]) (a subrange that
+//   may alias S), and for each u in R(x) its list T(u) =
+//   theads[t_beg[u] .. t_end[u]).  Every hit w in S(x) cap T(u) emits the
+//   triangle {pv_tail(x), u, w}.
+struct TcView {
+    uint32_t X;                 // number of tails
+    const uint64_t* s_off;      // X+1
+    const uint32_t* sheads;
+    const uint64_t* r_beg;      // X (nullptr: R = S)
+    const uint64_t* r_end;      // X
+    const uint32_t* rheads;     // (nullptr: = sheads)
+    const uint64_t* t_beg;      // per u (indexed by u)
+    const uint64_t* t_end;      // per u (nullptr: t_end[u] = t_beg[u+1])
+    const uint32_t* theads;
+    const uint32_t* tail_id;    // X (nullptr: identity) -> per-vertex index of the tail
+    const uint32_t* tail_graph; // X (nullptr: all graph 0) -> index into per-graph totals
+};
+
+struct TcOut {
+    uint64_t* per_vertex;       // nullable
+    uint64_t* totals;           // per graph (>= 1 entry)
+};
+
+uint64_t tc_ws_words(uint32_t X);
+void tc_launch(const TcView& view, const TcOut& out, const TcParams& prm, uint64_t* ws, int num_cus,
+               hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end);
+
+}  // namespace kt

@@ -1,0 +1,397 @@
+// Oriented-CSR construction on the device: degree orientation, prefix-scan
+// compaction, LSD radix sort (for unsorted inputs and ghost-list grouping).
+//
+// Replaces DistributedGraph::orient / sort_neighborhoods
+// (include/cetric/datastructures/distributed/distributed_graph.h:537-593) and
+// the out-only CSR rewrite of remove_in_edges_and_expand_ghosts (:1057-1169)
+// for the shared-memory path (counters/shared_memory_cetric.h:71-103).
+//
+// Layout: the symmetric input CSR is (off[n+1] u64, heads[nnz] u32).  The
+// oriented CSR is (ooff[n+1] u64, oheads[nnz/2] u32): for each v the heads u
+// with (deg v, v) < (deg u, u), in input order (ascending ids when the input
+// lists are sorted; otherwise they are sorted afterwards by radix sort).
+#include "kt_device.h"
+#include "kt_internal.h"
+
+namespace kt {
+
+// ---------------------------------------------------------------------------
+// generic block exclusive scan over 4096-element tiles (256 threads x 16)
+// ---------------------------------------------------------------------------
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+// Exclusive scan of one value per thread across a 256-thread block.
+__device__ __forceinline__ uint64_t block_excl_sum(uint64_t x, uint64_t* lds_wave, uint64_t* total) {
+    const unsigned w = threadIdx.x / KT_WAVE;
+    uint64_t incl = wave_incl_sum(x);
+    if (lane_id() == KT_WAVE - 1) lds_wave[w] = incl;
+    __syncthreads();
+    uint64_t base = 0, tot = 0;
+#pragma unroll
+    for (unsigned i = 0; i < SCAN_THREADS / KT_WAVE; ++i) {
+        uint64_t t = lds_wave[i];
+        if (i < w) base += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + incl - x;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_tile_reduce(const uint64_t* __restrict__ in, uint64_t len,
+                                                               uint64_t* __restrict__ sums) {
+    __shared__ uint64_t lw[SCAN_THREADS / KT_WAVE];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+        if (base + i < len) s += in[base + i];
+    uint64_t tot;
+    block_excl_sum(s, lw, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// out[i] = tile_base[tile] + exclusive prefix within the tile; out[len] = grand total.
+__global__ void __launch_bounds__(SCAN_THREADS) k_tile_scan(const uint64_t* __restrict__ in, uint64_t len,
+                                                             const uint64_t* __restrict__ tile_base,
+                                                             uint64_t* __restrict__ out) {
+    __shared__ uint64_t lw[SCAN_THREADS / KT_WAVE];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint64_t v[SCAN_ITEMS];
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = (base + i < len) ? in[base + i] : 0;
+        s += v[i];
+    }
+    uint64_t tot;
+    uint64_t run = block_excl_sum(s, lw, &tot) + (tile_base ? tile_base[blockIdx.x] : 0);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < len) out[base + i] = run;
+        run += v[i];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        out[len] = (tile_base ? tile_base[blockIdx.x] : 0) + tot;
+}
+
+uint64_t scan_tmp_words(uint64_t len) {
+    uint64_t words = 0;
+    while (len > SCAN_TILE) {
+        len = (len + SCAN_TILE - 1) / SCAN_TILE;
+        words += 2 * (len + 1);
+    }
+    return words + 2;
+}
+
+// Exclusive scan, out has len+1 entries (out[len] = total).  in may alias out.
+void scan_u64(const uint64_t* in, uint64_t* out, uint64_t len, uint64_t* tmp, hipStream_t st) {
+    if (len == 0) {
+        fill_u64(out, 1, 0, st);
+        return;
+    }
+    uint64_t tiles = (len + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles == 1) {
+        k_tile_scan<<<1, SCAN_THREADS, 0, st>>>(in, len, nullptr, out);
+        return;
+    }
+    uint64_t* sums = tmp;
+    uint64_t* sums_off = tmp + tiles + 1;
+    k_tile_reduce<<<(unsigned)tiles, SCAN_THREADS, 0, st>>>(in, len, sums);
+    scan_u64(sums, sums_off, tiles, tmp + 2 * (tiles + 1), st);
+    k_tile_scan<<<(unsigned)tiles, SCAN_THREADS, 0, st>>>(in, len, sums_off, out);
+}
+
+// ---------------------------------------------------------------------------
+// degrees + offset validation
+// ---------------------------------------------------------------------------
+__global__ void k_degree(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, uint32_t* __restrict__ deg,
+                         uint32_t* __restrict__ err) {
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v == 0 && (off[0] != 0 || off[n] != nnz)) atomicOr(err, KT_ERR_OFFSETS);
+    if (v >= n) return;
+    uint64_t a = off[v], b = off[v + 1];
+    if (b < a || b - a > 0xffffffffull) {
+        atomicOr(err, KT_ERR_OFFSETS);
+        deg[v] = 0;
+        return;
+    }
+    deg[v] = (uint32_t)(b - a);
+}
+
+// ---------------------------------------------------------------------------
+// orientation: count pass + write pass over 4096-slot tiles
+// ---------------------------------------------------------------------------
+constexpr int OR_THREADS = 256;
+constexpr int OR_ITEMS = 16;
+constexpr int OR_TILE = OR_THREADS * OR_ITEMS;
+
+// (deg v, v) < (deg u, u): node_ordering::degree_outward with every vertex
+// local (distributed_graph.h:147-163).
+__device__ __forceinline__ bool keep_edge(uint32_t v, uint32_t dv, uint32_t u, uint32_t du) {
+    return dv < du || (dv == du && v < u);
+}
+
+struct SlotRange {
+    uint64_t s0, s1;   // [s0, s1) slots of this thread
+    uint32_t row;      // row of s0
+    int64_t prev_row;  // row of s0-1 (-1 for s0 == 0)
+};
+
+__device__ __forceinline__ SlotRange thread_slots(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off,
+                                                  uint32_t* lds_bounds) {
+    const uint64_t tile0 = (uint64_t)blockIdx.x * OR_TILE;
+    if (threadIdx.x == 0) {
+        uint64_t last = min(tile0 + OR_TILE, nnz) - 1;
+        uint64_t first = tile0 > 0 ? tile0 - 1 : 0;
+        lds_bounds[0] = row_of_slot(off, 0, n - 1, first);
+        lds_bounds[1] = row_of_slot(off, 0, n - 1, last);
+    }
+    __syncthreads();
+    const uint32_t lo = lds_bounds[0], hi = lds_bounds[1];
+    SlotRange r;
+    r.s0 = tile0 + (uint64_t)threadIdx.x * OR_ITEMS;
+    r.s1 = min(r.s0 + OR_ITEMS, nnz);
+    if (r.s0 < nnz) {
+        r.row = row_of_slot(off, lo, hi, r.s0);
+        r.prev_row = r.s0 == 0 ? -1 : (int64_t)row_of_slot(off, lo, r.row, r.s0 - 1);
+    } else {
+        r.row = hi;
+        r.prev_row = hi;
+    }
+    return r;
+}
+
+__global__ void __launch_bounds__(OR_THREADS)
+    k_orient_count(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
+                   const uint32_t* __restrict__ deg, uint64_t* __restrict__ tile_cnt, uint32_t* __restrict__ err) {
+    __shared__ uint32_t bounds[2];
+    __shared__ uint64_t lw[OR_THREADS / KT_WAVE];
+    SlotRange r = thread_slots(n, nnz, off, bounds);
+    uint32_t cnt = 0, flags = 0;
+    uint32_t cur = r.row;
+    for (uint64_t s = r.s0; s < r.s1; ++s) {
+        while (cur + 1 < n && off[cur + 1] <= s) ++cur;
+        uint32_t u = heads[s];
+        if (u >= n) {
+            flags |= KT_ERR_HEAD_RANGE;
+            continue;
+        }
+        if (u == cur) flags |= KT_ERR_SELF_LOOP;
+        if (s + 1 < nnz && s + 1 < off[cur + 1]) {
+            uint32_t nx = heads[s + 1];
+            if (nx == u)
+                flags |= KT_ERR_DUPLICATE;
+            else if (nx < u)
+                flags |= KT_FLAG_UNSORTED;
+        }
+        cnt += keep_edge(cur, deg[cur], u, deg[u]) ? 1u : 0u;
+    }
+    if (flags) atomicOr(err, flags);
+    uint64_t tot;
+    block_excl_sum(cnt, lw, &tot);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
+}
+
+// Writes oriented heads (or (row<<32|head) keys when keys != nullptr, for the
+// unsorted path) and the oriented offsets of every row whose first slot lies
+// in this tile.
+__global__ void __launch_bounds__(OR_THREADS)
+    k_orient_write(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
+                   const uint32_t* __restrict__ deg, const uint64_t* __restrict__ tile_off,
+                   uint64_t* __restrict__ ooff, uint32_t* __restrict__ oheads, uint64_t* __restrict__ keys) {
+    __shared__ uint32_t bounds[2];
+    __shared__ uint64_t lw[OR_THREADS / KT_WAVE];
+    SlotRange r = thread_slots(n, nnz, off, bounds);
+    // pass A: count own kept slots
+    uint32_t cnt = 0;
+    {
+        uint32_t cur = r.row;
+        for (uint64_t s = r.s0; s < r.s1; ++s) {
+            while (cur + 1 < n && off[cur + 1] <= s) ++cur;
+            uint32_t u = heads[s];
+            if (u < n && keep_edge(cur, deg[cur], u, deg[u])) ++cnt;
+        }
+    }
+    uint64_t tot;
+    uint64_t pos = block_excl_sum(cnt, lw, &tot) + tile_off[blockIdx.x];
+    // pass B: write
+    uint32_t cur = r.row;
+    int64_t prev = r.prev_row;
+    for (uint64_t s = r.s0; s < r.s1; ++s) {
+        while (cur + 1 < n && off[cur + 1] <= s) ++cur;
+        for (int64_t q = prev + 1; q <= (int64_t)cur; ++q) ooff[q] = pos;  // rows starting at slot s
+        prev = cur;
+        uint32_t u = heads[s];
+        if (u < n && keep_edge(cur, deg[cur], u, deg[u])) {
+            if (keys)
+                keys[pos] = ((uint64_t)cur << 32) | u;
+            else
+                oheads[pos] = u;
+            ++pos;
+        }
+    }
+}
+
+// Rows whose first slot is nnz (trailing empty rows and row n).
+__global__ void k_orient_tail(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off,
+                              const uint64_t* __restrict__ total, uint64_t* __restrict__ ooff) {
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v <= n && (v == n || off[v] >= nnz)) ooff[v] = *total;
+}
+
+// After sorting (row<<32|head) keys: extract heads, flag duplicates.
+__global__ void k_keys_to_heads(uint64_t m, const uint64_t* __restrict__ keys, uint32_t* __restrict__ oheads,
+                                uint32_t* __restrict__ err) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    uint64_t k = keys[i];
+    oheads[i] = (uint32_t)k;
+    if (i + 1 < m && keys[i + 1] == k) atomicOr(err, KT_ERR_DUPLICATE);
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of u64 keys, 8-bit digits, stable
+// ---------------------------------------------------------------------------
+constexpr int RS_THREADS = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint64_t* __restrict__ keys, uint64_t len,
+                                                            int shift, uint64_t* __restrict__ hist,
+                                                            uint32_t ntiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    for (int i = 0; i < RS_ROUNDS; ++i) {
+        uint64_t idx = base + (uint64_t)i * RS_THREADS + threadIdx.x;
+        if (idx < len) atomicAdd(&h[(keys[idx] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const uint64_t* __restrict__ in, uint64_t len,
+                                                               int shift, const uint64_t* __restrict__ hist_off,
+                                                               uint32_t ntiles, uint64_t* __restrict__ out) {
+    __shared__ uint64_t run[256];
+    __shared__ uint32_t wcnt[RS_THREADS / KT_WAVE][256];
+    const unsigned w = threadIdx.x / KT_WAVE;
+    run[threadIdx.x] = hist_off[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    for (int i = 0; i < RS_ROUNDS; ++i) {
+        for (int q = 0; q < RS_THREADS / KT_WAVE; ++q) wcnt[q][threadIdx.x] = 0;
+        __syncthreads();
+        uint64_t idx = base + (uint64_t)i * RS_THREADS + threadIdx.x;
+        bool valid = idx < len;
+        uint64_t k = valid ? in[idx] : 0;
+        uint32_t d = (uint32_t)((k >> shift) & 255);
+        // lanes of this wave holding the same digit
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t bb = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? bb : ~bb;
+        }
+        uint32_t wrank = __popcll(peers & lanemask_lt());
+        bool leader = valid && wrank == 0;
+        if (leader) wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        {
+            uint64_t r = run[threadIdx.x];
+#pragma unroll
+            for (int q = 0; q < RS_THREADS / KT_WAVE; ++q) {
+                uint32_t c = wcnt[q][threadIdx.x];
+                wcnt[q][threadIdx.x] = (uint32_t)(r - run[threadIdx.x]);
+                r += c;
+            }
+            __syncthreads();
+            if (valid) out[run[d] + wcnt[w][d] + wrank] = k;
+            __syncthreads();
+            run[threadIdx.x] = r;
+        }
+        __syncthreads();
+    }
+}
+
+uint64_t radix_tmp_words(uint64_t len) {
+    uint64_t ntiles = (len + RS_TILE - 1) / RS_TILE;
+    uint64_t h = 256 * ntiles;
+    return 2 * (h + 1) + scan_tmp_words(h);
+}
+
+// Sorts keys[0..len) by bits [0, bits); result ends in `keys` (alt is scratch
+// of len words, tmp of radix_tmp_words(len) words).
+void radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t len, int bits, uint64_t* tmp, hipStream_t st) {
+    if (len <= 1) return;
+    uint32_t ntiles = (uint32_t)((len + RS_TILE - 1) / RS_TILE);
+    uint64_t h = 256ull * ntiles;
+    uint64_t* hist = tmp;
+    uint64_t* hoff = tmp + h + 1;
+    uint64_t* stmp = tmp + 2 * (h + 1);
+    int passes = (bits + 7) / 8;
+    uint64_t* src = keys;
+    uint64_t* dst = alt;
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        k_radix_hist<<<ntiles, RS_THREADS, 0, st>>>(src, len, shift, hist, ntiles);
+        scan_u64(hist, hoff, h, stmp, st);
+        k_radix_scatter<<<ntiles, RS_THREADS, 0, st>>>(src, len, shift, hoff, ntiles, dst);
+        uint64_t* t = src;
+        src = dst;
+        dst = t;
+    }
+    if (src != keys) hipMemcpyAsync(keys, src, len * sizeof(uint64_t), hipMemcpyDeviceToDevice, st);
+}
+
+// ---------------------------------------------------------------------------
+// host driver for the plain orientation
+// ---------------------------------------------------------------------------
+uint64_t orient_ws_words(uint32_t n, uint64_t nnz) {
+    uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
+    return 2 * (tiles + 1) + scan_tmp_words(tiles + 1);
+}
+
+// Degrees, validation flags, kept-edge count per tile and its scan; the
+// grand total lands in ws[tiles + 1 + tiles] (= tile_off[tiles]).
+void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,
+                         uint64_t* ws, uint32_t* err, hipStream_t st) {
+    uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
+    uint64_t* tile_cnt = ws;
+    uint64_t* tile_off = ws + tiles + 1;
+    uint64_t* stmp = ws + 2 * (tiles + 1);
+    k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, err);
+    if (tiles > 0) {
+        k_orient_count<<<(unsigned)tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, tile_cnt, err);
+        scan_u64(tile_cnt, tile_off, tiles, stmp, st);
+    } else {
+        fill_u64(tile_off, 1, 0, st);
+    }
+}
+
+const uint64_t* orient_total_ptr(uint64_t nnz, const uint64_t* ws) {
+    uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
+    return ws + tiles + 1 + tiles;
+}
+
+// Writes the oriented CSR; only valid after orient_count_launch on the same
+// input showed a symmetric, well-formed CSR (kept == nnz / 2).
+void orient_write_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, const uint32_t* deg,
+                         uint64_t* ooff, uint32_t* oheads, uint64_t* keys, const uint64_t* ws, hipStream_t st) {
+    uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
+    const uint64_t* tile_off = ws + tiles + 1;
+    if (tiles > 0)
+        k_orient_write<<<(unsigned)tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, tile_off, ooff, oheads,
+                                                               keys);
+    k_orient_tail<<<(n + 1 + 255) / 256, 256, 0, st>>>(n, nnz, off, tile_off + tiles, ooff);
+}
+
+void keys_to_heads(uint64_t m, const uint64_t* keys, uint32_t* oheads, uint32_t* err, hipStream_t st) {
+    if (m == 0) return;
+    k_keys_to_heads<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(m, keys, oheads, err);
+}
+
+}  // namespace kt
