@@ -76,9 +76,9 @@ typedef struct {
 /* Intersection strategy knobs (bin thresholds); see kt_tune_* in the
  * package for the autotuner that chooses them. */
 typedef struct {
-    uint32_t wave_max_deg;  /* |N+(v)| <= this: one wave per tail, LDS hash (<= 512) */
+    uint32_t wave_max_deg;  /* |N+(v)| <= this: one wave per tail, LDS hash (<= 256) */
     uint32_t group_max_deg; /* |N+(v)| <= this: one workgroup per tail, LDS hash (<= 2048) */
-    uint32_t grid_per_cu;   /* persistent workgroups per compute unit */
+    uint32_t grid_per_cu;   /* persistent 512-thread workgroups per compute unit */
     uint32_t reserved;
 } kt_params;
 

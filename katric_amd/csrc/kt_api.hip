@@ -146,9 +146,9 @@ extern "C" int kt_create(kt_ctx** out, int device, uint64_t hbm_budget_bytes) {
         kt_destroy(c);
         return KT_EHIP;
     }
-    c->prm.wave_max_deg = 512;
+    c->prm.wave_max_deg = 128;
     c->prm.group_max_deg = 2048;
-    c->prm.split_chunks = 0;
+    c->prm.team_chunk = 16;
     c->prm.grid_per_cu = 4;
     trace("context on device %d: %d CUs, stream %p", device, c->num_cus, (void*)c->stream);
     *out = c;
@@ -188,7 +188,7 @@ extern "C" const char* kt_last_error(const kt_ctx* c) { return c ? c->err : "no 
 
 extern "C" int kt_set_params(kt_ctx* c, const kt_params* p) {
     if (!c || !p) return KT_EINVAL;
-    if (p->wave_max_deg < 2 || p->wave_max_deg > 512 || p->group_max_deg < p->wave_max_deg ||
+    if (p->wave_max_deg < 2 || p->wave_max_deg > 256 || p->group_max_deg < p->wave_max_deg ||
         p->group_max_deg > 2048 || p->grid_per_cu < 1 || p->grid_per_cu > 32)
         return KT_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -255,7 +255,7 @@ uint64_t plain_layout(void* base, uint32_t n, uint64_t nnz, bool host_in, bool w
     }
     B.deg = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(n, 1) * 4));
     B.ooff = static_cast<uint64_t*>(cv.take((uint64_t(n) + 1) * 8));
-    B.oheads = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(m, 1) * 4));
+    B.oheads = static_cast<uint32_t*>(cv.take((m + 4) * 4));  // +4: 16-byte block reads past the end
     B.ws_orient = static_cast<uint64_t*>(cv.take(orient_ws_words(n, nnz) * 8));
     B.ws_tc = static_cast<uint64_t*>(cv.take(tc_ws_words(n) * 8));
     B.pv = want_pv_dev ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(n, 1) * 8)) : nullptr;
@@ -297,12 +297,11 @@ int upload_and_count(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_o
     hipEventRecord(c->ev[EV_H2D], st);
     trace("upload done, launching count pass");
     fill_u64(B.small, 64, 0, st);
-    orient_count_launch(n, nnz, B.off, B.heads, B.deg, B.ws_orient, reinterpret_cast<uint32_t*>(B.small), st);
+    orient_count_launch(n, nnz, B.off, B.heads, B.deg, B.ooff, B.ws_orient, reinterpret_cast<uint32_t*>(B.small),
+                        st);
     rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st), "D2H flags");
     if (rc) return rc;
-    rc = c->hip_check(
-        hipMemcpyAsync(c->host_small + 1, orient_total_ptr(nnz, B.ws_orient), 8, hipMemcpyDeviceToHost, st),
-        "D2H kept");
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 1, B.ooff + n, 8, hipMemcpyDeviceToHost, st), "D2H kept");
     if (rc) return rc;
     return c->hip_check(hipStreamSynchronize(st), "orientation count");
 }
@@ -354,7 +353,8 @@ int run_orientation(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_of
                 B.heads = user_heads;
             }
         }
-        orient_write_launch(n, nnz, B.off, B.heads, B.deg, B.ooff, nullptr, B.keys, B.ws_orient, st);
+        orient_write_launch(n, nnz, B.heads, B.oheads, B.ws_orient, st);
+        make_keys(n, m, B.ooff, B.oheads, B.keys, st);
         radix_sort_u64(B.keys, B.alt, m, 32 + bits_for(n), B.ws_radix, st);
         keys_to_heads(m, B.keys, B.oheads, reinterpret_cast<uint32_t*>(B.small), st);
         rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st), "D2H flags");
@@ -364,7 +364,7 @@ int run_orientation(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_of
         flags = static_cast<uint32_t>(c->host_small[0]);
         if (flags & KT_ERR_MASK) return c->fail(KT_EINVAL, "malformed CSR: duplicate edge");
     } else {
-        orient_write_launch(n, nnz, B.off, B.heads, B.deg, B.ooff, B.oheads, nullptr, B.ws_orient, st);
+        orient_write_launch(n, nnz, B.heads, B.oheads, B.ws_orient, st);
     }
     c->stats.oriented_edges = kept;
     c->stats.vertices = n;

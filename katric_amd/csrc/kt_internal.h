@@ -23,10 +23,11 @@ uint64_t radix_tmp_words(uint64_t len);
 void radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t len, int bits, uint64_t* tmp, hipStream_t st);
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz);
 void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,
-                         uint64_t* ws, uint32_t* err, hipStream_t st);
-const uint64_t* orient_total_ptr(uint64_t nnz, const uint64_t* ws);
-void orient_write_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, const uint32_t* deg,
-                         uint64_t* ooff, uint32_t* oheads, uint64_t* keys, const uint64_t* ws, hipStream_t st);
+                         uint64_t* ooff, uint64_t* ws, uint32_t* err, hipStream_t st);
+void orient_write_launch(uint32_t n, uint64_t nnz, const uint32_t* heads, uint32_t* oheads, uint64_t* ws,
+                         hipStream_t st);
+void make_keys(uint32_t n, uint64_t m, const uint64_t* ooff, const uint32_t* oheads, uint64_t* keys,
+               hipStream_t st);
 void keys_to_heads(uint64_t m, const uint64_t* keys, uint32_t* oheads, uint32_t* err, hipStream_t st);
 
 // fill kernels (kt_util.hip); used instead of hipMemsetAsync so that every
@@ -38,7 +39,7 @@ void fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t st);
 struct TcParams {
     uint32_t wave_max_deg;    // d+(v) <= this: one wave per tail vertex (LDS hash)
     uint32_t group_max_deg;   // d+(v) <= this: one workgroup (4 waves) per tail vertex (LDS hash)
-    uint32_t split_chunks;    // >0: tails with more than this many 64-neighbour chunks are split
+    uint32_t team_chunk;      // neighbours per work grab inside a workgroup team (1..64)
     uint32_t grid_per_cu;     // persistent workgroups per CU
 };
 

@@ -10,6 +10,8 @@
 // oriented CSR is (ooff[n+1] u64, oheads[nnz/2] u32): for each v the heads u
 // with (deg v, v) < (deg u, u), in input order (ascending ids when the input
 // lists are sorted; otherwise they are sorted afterwards by radix sort).
+#include <algorithm>
+
 #include "kt_device.h"
 #include "kt_internal.h"
 
@@ -122,7 +124,16 @@ __global__ void k_degree(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// orientation: count pass + write pass over 4096-slot tiles
+// orientation over 4096-slot tiles of the symmetric CSR
+//
+// k_orient_keep: slots are visited coalesced (slot = tile0 + k*256 + tid).
+// The row of every slot is resolved in LDS: each row starting inside the tile
+// marks its first slot, and an inclusive max-scan over the 4096 marks gives
+// the row of every slot.  The keep predicate (deg v, v) < (deg u, u) is
+// evaluated once per slot and stored as a bitmask (one u64 word per 64
+// slots); per-row kept counts d+(v) and the per-tile kept count are written
+// for the scans.  k_orient_emit then copies the kept heads in slot order using
+// only the bitmask and the tile bases.
 // ---------------------------------------------------------------------------
 constexpr int OR_THREADS = 256;
 constexpr int OR_ITEMS = 16;
@@ -134,112 +145,159 @@ __device__ __forceinline__ bool keep_edge(uint32_t v, uint32_t dv, uint32_t u, u
     return dv < du || (dv == du && v < u);
 }
 
-struct SlotRange {
-    uint64_t s0, s1;   // [s0, s1) slots of this thread
-    uint32_t row;      // row of s0
-    int64_t prev_row;  // row of s0-1 (-1 for s0 == 0)
-};
+__device__ __forceinline__ uint32_t wave_incl_max_u32(uint32_t x) {
+    unsigned l = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (l >= (unsigned)d) x = max(x, y);
+    }
+    return x;
+}
 
-__device__ __forceinline__ SlotRange thread_slots(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off,
-                                                  uint32_t* lds_bounds) {
+__global__ void __launch_bounds__(OR_THREADS)
+    k_orient_keep(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
+                  const uint32_t* __restrict__ deg, uint64_t* __restrict__ keep_bits,
+                  uint64_t* __restrict__ dplus, uint64_t* __restrict__ tile_cnt, uint32_t* __restrict__ err) {
+    __shared__ uint32_t marks[OR_TILE];
+    __shared__ uint32_t rowcnt[OR_TILE];
+    __shared__ uint32_t wmax[OR_THREADS / KT_WAVE];
+    __shared__ uint32_t bnd[2];
+    __shared__ uint64_t lw[OR_THREADS / KT_WAVE];
     const uint64_t tile0 = (uint64_t)blockIdx.x * OR_TILE;
-    if (threadIdx.x == 0) {
-        uint64_t last = min(tile0 + OR_TILE, nnz) - 1;
-        uint64_t first = tile0 > 0 ? tile0 - 1 : 0;
-        lds_bounds[0] = row_of_slot(off, 0, n - 1, first);
-        lds_bounds[1] = row_of_slot(off, 0, n - 1, last);
+    const uint64_t tend = min(tile0 + OR_TILE, nnz);
+    const unsigned tid = threadIdx.x;
+    if (tid == 0) {
+        bnd[0] = row_of_slot(off, 0, n - 1, tile0);
+        bnd[1] = row_of_slot(off, 0, n - 1, tend - 1);
+    }
+    for (int i = tid; i < OR_TILE; i += OR_THREADS) {
+        marks[i] = 0;
+        rowcnt[i] = 0;
     }
     __syncthreads();
-    const uint32_t lo = lds_bounds[0], hi = lds_bounds[1];
-    SlotRange r;
-    r.s0 = tile0 + (uint64_t)threadIdx.x * OR_ITEMS;
-    r.s1 = min(r.s0 + OR_ITEMS, nnz);
-    if (r.s0 < nnz) {
-        r.row = row_of_slot(off, lo, hi, r.s0);
-        r.prev_row = r.s0 == 0 ? -1 : (int64_t)row_of_slot(off, lo, r.row, r.s0 - 1);
-    } else {
-        r.row = hi;
-        r.prev_row = hi;
+    const uint32_t lo = bnd[0], hi = bnd[1];
+    const uint64_t nrows = (uint64_t)hi - lo + 1;
+    const bool lds_rows = nrows <= OR_TILE;
+    // rows starting strictly inside the tile mark their first slot
+    for (uint64_t r = (uint64_t)lo + 1 + tid; r <= hi; r += OR_THREADS) {
+        uint64_t pos = off[r] - tile0;
+        if (pos < OR_TILE) atomicMax(&marks[pos], (uint32_t)(r - lo));
     }
-    return r;
-}
-
-__global__ void __launch_bounds__(OR_THREADS)
-    k_orient_count(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
-                   const uint32_t* __restrict__ deg, uint64_t* __restrict__ tile_cnt, uint32_t* __restrict__ err) {
-    __shared__ uint32_t bounds[2];
-    __shared__ uint64_t lw[OR_THREADS / KT_WAVE];
-    SlotRange r = thread_slots(n, nnz, off, bounds);
-    uint32_t cnt = 0, flags = 0;
-    uint32_t cur = r.row;
-    for (uint64_t s = r.s0; s < r.s1; ++s) {
-        while (cur + 1 < n && off[cur + 1] <= s) ++cur;
-        uint32_t u = heads[s];
-        if (u >= n) {
-            flags |= KT_ERR_HEAD_RANGE;
-            continue;
+    __syncthreads();
+    // inclusive max-scan over the 4096 marks (16 consecutive per thread)
+    {
+        uint32_t m = 0;
+#pragma unroll
+        for (int i = 0; i < OR_ITEMS; ++i) m = max(m, marks[tid * OR_ITEMS + i]);
+        uint32_t incl = wave_incl_max_u32(m);
+        if (lane_id() == KT_WAVE - 1) wmax[tid / KT_WAVE] = incl;
+        __syncthreads();
+        uint32_t carry = __shfl_up(incl, 1, 64);
+        if (lane_id() == 0) carry = 0;
+        for (unsigned w = 0; w < tid / KT_WAVE; ++w) carry = max(carry, wmax[w]);
+        uint32_t run = carry;
+#pragma unroll
+        for (int i = 0; i < OR_ITEMS; ++i) {
+            run = max(run, marks[tid * OR_ITEMS + i]);
+            marks[tid * OR_ITEMS + i] = run;
         }
-        if (u == cur) flags |= KT_ERR_SELF_LOOP;
-        if (s + 1 < nnz && s + 1 < off[cur + 1]) {
-            uint32_t nx = heads[s + 1];
-            if (nx == u)
-                flags |= KT_ERR_DUPLICATE;
-            else if (nx < u)
-                flags |= KT_FLAG_UNSORTED;
+    }
+    __syncthreads();
+    uint32_t flags = 0, cnt = 0;
+#pragma unroll 4
+    for (int k = 0; k < OR_ITEMS; ++k) {
+        const unsigned i = k * OR_THREADS + tid;
+        const uint64_t sl = tile0 + i;
+        bool keep = false;
+        if (sl < tend) {
+            const uint32_t rr = marks[i];
+            const uint32_t v = lo + rr;
+            const uint32_t u = heads[sl];
+            if (u >= n) {
+                flags |= KT_ERR_HEAD_RANGE;
+            } else {
+                if (u == v) flags |= KT_ERR_SELF_LOOP;
+                keep = keep_edge(v, deg[v], u, deg[u]);
+            }
+            // next slot in the same row?
+            const bool same = (i + 1 < OR_TILE) ? (sl + 1 < tend && marks[i + 1] == rr)
+                                                 : (sl + 1 < nnz && sl + 1 < off[v + 1]);
+            if (same) {
+                const uint32_t nx = heads[sl + 1];
+                if (nx == u)
+                    flags |= KT_ERR_DUPLICATE;
+                else if (nx < u)
+                    flags |= KT_FLAG_UNSORTED;
+            }
+            if (keep) {
+                ++cnt;
+                if (lds_rows)
+                    atomicAdd(&rowcnt[rr], 1u);
+                else
+                    atomic_add_u64(&dplus[v], 1);
+            }
         }
-        cnt += keep_edge(cur, deg[cur], u, deg[u]) ? 1u : 0u;
+        const uint64_t bits = __ballot(keep);
+        if (lane_id() == 0 && tile0 + k * OR_THREADS + (tid & ~63u) < tend)
+            keep_bits[(tile0 + k * OR_THREADS + (tid & ~63u)) / 64] = bits;
     }
     if (flags) atomicOr(err, flags);
+    __syncthreads();
+    if (lds_rows) {
+        // rows wholly inside the tile get a plain store, the two boundary rows an atomic
+        for (uint64_t rr = tid; rr < nrows; rr += OR_THREADS) {
+            const uint32_t c = rowcnt[rr];
+            const uint32_t v = lo + (uint32_t)rr;
+            if (rr == 0 || rr == nrows - 1) {
+                if (c) atomic_add_u64(&dplus[v], c);
+            } else {
+                dplus[v] = c;
+            }
+        }
+    }
     uint64_t tot;
     block_excl_sum(cnt, lw, &tot);
-    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
+    if (tid == 0) tile_cnt[blockIdx.x] = tot;
 }
 
-// Writes oriented heads (or (row<<32|head) keys when keys != nullptr, for the
-// unsorted path) and the oriented offsets of every row whose first slot lies
-// in this tile.
+// Copies the kept heads in slot order: output position = tile base + rank of
+// the slot among the tile's kept slots.
 __global__ void __launch_bounds__(OR_THREADS)
-    k_orient_write(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
-                   const uint32_t* __restrict__ deg, const uint64_t* __restrict__ tile_off,
-                   uint64_t* __restrict__ ooff, uint32_t* __restrict__ oheads, uint64_t* __restrict__ keys) {
-    __shared__ uint32_t bounds[2];
-    __shared__ uint64_t lw[OR_THREADS / KT_WAVE];
-    SlotRange r = thread_slots(n, nnz, off, bounds);
-    // pass A: count own kept slots
-    uint32_t cnt = 0;
-    {
-        uint32_t cur = r.row;
-        for (uint64_t s = r.s0; s < r.s1; ++s) {
-            while (cur + 1 < n && off[cur + 1] <= s) ++cur;
-            uint32_t u = heads[s];
-            if (u < n && keep_edge(cur, deg[cur], u, deg[u])) ++cnt;
+    k_orient_emit(uint64_t nnz, const uint32_t* __restrict__ heads, const uint64_t* __restrict__ keep_bits,
+                  const uint64_t* __restrict__ tile_off, uint32_t* __restrict__ oheads) {
+    __shared__ uint32_t wcnt[OR_THREADS / KT_WAVE];
+    const uint64_t tile0 = (uint64_t)blockIdx.x * OR_TILE;
+    const unsigned tid = threadIdx.x, wave = tid / KT_WAVE;
+    uint64_t base = tile_off[blockIdx.x];
+    for (int k = 0; k < OR_ITEMS; ++k) {
+        const uint64_t w0 = tile0 + k * OR_THREADS + wave * KT_WAVE;
+        const uint64_t bits = (w0 < nnz) ? keep_bits[w0 / 64] : 0;
+        if (lane_id() == 0) wcnt[wave] = (uint32_t)__popcll(bits);
+        __syncthreads();
+        uint32_t before = 0, round = 0;
+#pragma unroll
+        for (unsigned w = 0; w < OR_THREADS / KT_WAVE; ++w) {
+            const uint32_t c = wcnt[w];
+            before += (w < wave) ? c : 0;
+            round += c;
         }
-    }
-    uint64_t tot;
-    uint64_t pos = block_excl_sum(cnt, lw, &tot) + tile_off[blockIdx.x];
-    // pass B: write
-    uint32_t cur = r.row;
-    int64_t prev = r.prev_row;
-    for (uint64_t s = r.s0; s < r.s1; ++s) {
-        while (cur + 1 < n && off[cur + 1] <= s) ++cur;
-        for (int64_t q = prev + 1; q <= (int64_t)cur; ++q) ooff[q] = pos;  // rows starting at slot s
-        prev = cur;
-        uint32_t u = heads[s];
-        if (u < n && keep_edge(cur, deg[cur], u, deg[u])) {
-            if (keys)
-                keys[pos] = ((uint64_t)cur << 32) | u;
-            else
-                oheads[pos] = u;
-            ++pos;
+        if ((bits >> lane_id()) & 1) {
+            const uint64_t pos = base + before + __popcll(bits & lanemask_lt());
+            oheads[pos] = heads[w0 + lane_id()];
         }
+        base += round;
+        __syncthreads();
     }
 }
 
-// Rows whose first slot is nnz (trailing empty rows and row n).
-__global__ void k_orient_tail(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off,
-                              const uint64_t* __restrict__ total, uint64_t* __restrict__ ooff) {
-    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v <= n && (v == n || off[v] >= nnz)) ooff[v] = *total;
+// Rebuilds (row<<32 | head) keys of the oriented CSR (unsorted input path).
+__global__ void k_make_keys(uint32_t n, uint64_t m, const uint64_t* __restrict__ ooff,
+                            const uint32_t* __restrict__ oheads, uint64_t* __restrict__ keys) {
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    uint32_t r = row_of_slot(ooff, 0, n - 1, p);
+    keys[p] = ((uint64_t)r << 32) | oheads[p];
 }
 
 // After sorting (row<<32|head) keys: extract heads, flag duplicates.
@@ -344,49 +402,67 @@ void radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t len, int bits, uint6
         src = dst;
         dst = t;
     }
-    if (src != keys) hipMemcpyAsync(keys, src, len * sizeof(uint64_t), hipMemcpyDeviceToDevice, st);
+    if (src != keys) (void)hipMemcpyAsync(keys, src, len * sizeof(uint64_t), hipMemcpyDeviceToDevice, st);
 }
 
 // ---------------------------------------------------------------------------
 // host driver for the plain orientation
 // ---------------------------------------------------------------------------
+struct OrientWs {
+    uint64_t tiles;
+    uint64_t* tile_cnt;  // tiles + 1
+    uint64_t* tile_off;  // tiles + 1 (tile_off[tiles] = kept total)
+    uint64_t* bits;      // ceil(nnz / 64) + 1
+    uint64_t* dplus;     // n + 1
+    uint64_t* stmp;
+};
+
+static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
+    OrientWs w;
+    w.tiles = (nnz + OR_TILE - 1) / OR_TILE;
+    w.tile_cnt = ws;
+    w.tile_off = w.tile_cnt + w.tiles + 1;
+    w.bits = w.tile_off + w.tiles + 1;
+    w.dplus = w.bits + (nnz + 63) / 64 + 1;
+    w.stmp = w.dplus + (uint64_t)n + 1;
+    return w;
+}
+
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz) {
     uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
-    return 2 * (tiles + 1) + scan_tmp_words(tiles + 1);
+    return 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 +
+           std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1));
 }
 
-// Degrees, validation flags, kept-edge count per tile and its scan; the
-// grand total lands in ws[tiles + 1 + tiles] (= tile_off[tiles]).
+// Degrees, validation flags, keep bitmask, d+ per row and its scan into ooff
+// (ooff[n] = kept total), per-tile kept counts and their scan.
 void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,
-                         uint64_t* ws, uint32_t* err, hipStream_t st) {
-    uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
-    uint64_t* tile_cnt = ws;
-    uint64_t* tile_off = ws + tiles + 1;
-    uint64_t* stmp = ws + 2 * (tiles + 1);
+                         uint64_t* ooff, uint64_t* ws, uint32_t* err, hipStream_t st) {
+    OrientWs w = orient_ws(n, nnz, ws);
     k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, err);
-    if (tiles > 0) {
-        k_orient_count<<<(unsigned)tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, tile_cnt, err);
-        scan_u64(tile_cnt, tile_off, tiles, stmp, st);
+    fill_u64(w.dplus, (uint64_t)n + 1, 0, st);
+    if (w.tiles > 0) {
+        k_orient_keep<<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.bits, w.dplus,
+                                                               w.tile_cnt, err);
+        scan_u64(w.tile_cnt, w.tile_off, w.tiles, w.stmp, st);
     } else {
-        fill_u64(tile_off, 1, 0, st);
+        fill_u64(w.tile_off, 1, 0, st);
     }
+    scan_u64(w.dplus, ooff, n, w.stmp, st);
 }
 
-const uint64_t* orient_total_ptr(uint64_t nnz, const uint64_t* ws) {
-    uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
-    return ws + tiles + 1 + tiles;
-}
-
-// Writes the oriented CSR; only valid after orient_count_launch on the same
+// Writes the oriented heads; only valid after orient_count_launch on the same
 // input showed a symmetric, well-formed CSR (kept == nnz / 2).
-void orient_write_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, const uint32_t* deg,
-                         uint64_t* ooff, uint32_t* oheads, uint64_t* keys, const uint64_t* ws, hipStream_t st) {
-    uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
-    const uint64_t* tile_off = ws + tiles + 1;
-    if (tiles > 0)
-        k_orient_write<<<(unsigned)tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, tile_off, ooff, oheads,
-                                                               keys);
-    k_orient_tail<<<(n + 1 + 255) / 256, 256, 0, st>>>(n, nnz, off, tile_off + tiles, ooff);
+void orient_write_launch(uint32_t n, uint64_t nnz, const uint32_t* heads, uint32_t* oheads, uint64_t* ws,
+                         hipStream_t st) {
+    OrientWs w = orient_ws(n, nnz, ws);
+    if (w.tiles > 0) k_orient_emit<<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(nnz, heads, w.bits, w.tile_off, oheads);
+}
+
+void make_keys(uint32_t n, uint64_t m, const uint64_t* ooff, const uint32_t* oheads, uint64_t* keys,
+               hipStream_t st) {
+    if (m == 0) return;
+    k_make_keys<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(n, m, ooff, oheads, keys);
 }
 
 void keys_to_heads(uint64_t m, const uint64_t* keys, uint32_t* oheads, uint32_t* err, hipStream_t st) {

@@ -4,43 +4,46 @@
 // merge_intersection (counters/intersection.h:40-96) with a degree-binned
 // hash-probe design for CDNA (wave64, LDS):
 //
-//   * a "team" (one wave, or the four waves of a workgroup for heavier tails)
-//     takes a tail x and builds a linear-probing hash of S(x) = N+(x) in LDS;
-//   * the team walks R(x) = N+(x) in chunks of 64 neighbours u; the lists
-//     T(u) = N+(u) of a chunk are concatenated and probed 64 elements per
-//     step, one per lane (load-balanced by a marker/max-scan owner search),
-//     so short and long T(u) lists both keep all lanes busy;
-//   * every hit w is one emitted Triangle{x, u, w}; its per-vertex counts are
-//     privatised -- per-slot LDS counters for w, per-lane counters for u, a
-//     register for x -- and flushed with one global atomic per non-zero
-//     counter.
+//   * a "team" -- one wave, or all eight waves of a 512-thread workgroup for
+//     heavier tails -- takes a tail x and builds a linear-probing hash of
+//     S(x) = N+(x) in LDS (load factor <= 1/2, <= 1/4 where the table allows);
+//   * the team walks R(x) = N+(x) one neighbour u per wave at a time (u, and
+//     the bounds of T(u) = N+(u), are wave-uniform scalar loads); the wave
+//     streams T(u) with coalesced 16-byte loads, four ids per lane, the next
+//     block prefetched while the current one is probed;
+//   * every hit w is one emitted Triangle{x, u, w}; per-vertex counts are
+//     privatised -- per-slot LDS counters for w, a wave sum for u, a team sum
+//     for x -- and flushed with one global atomic per non-zero counter;
+//   * tails are bucketed by ceil(log2 |S(x)|) and taken largest first from
+//     per-bucket work queues (dynamic, LPT-like balance across the grid);
 //   * tails whose S(x) exceeds the LDS table are probed by binary search in
 //     the sorted S(x) in global memory instead.
 //
 // Counts are exact integers; the hash only changes where a set member is
-// stored, never which members match.
+// stored, never which members match.  T lists are read in aligned 16-byte
+// blocks and may be touched up to 3 ids past their end: every heads array
+// handed to this kernel carries 4 ids of padding.
+#include <algorithm>
+
 #include "kt_device.h"
 #include "kt_internal.h"
 
 namespace kt {
 
-constexpr int TC_THREADS = 256;
-constexpr int TC_WAVES = TC_THREADS / KT_WAVE;
-constexpr uint32_t GROUP_SLOTS = 4096;               // shared table of a 4-wave team
-constexpr uint32_t WAVE_SLOTS = GROUP_SLOTS / TC_WAVES;  // private table of a 1-wave team
+constexpr int TC_THREADS = 512;
+constexpr int TC_WAVES = TC_THREADS / KT_WAVE;           // 8
+constexpr uint32_t GROUP_SLOTS = 4096;                   // shared table of the 8-wave team
+constexpr uint32_t WAVE_SLOTS = GROUP_SLOTS / TC_WAVES;  // private table of a 1-wave team (512)
 constexpr uint32_t MIN_LOGS = 6;
+constexpr int NBUCKET = 13;  // bucket b holds tails with |S| in (2^(b-1), 2^b]; bucket 12 = huge
+constexpr int BUCKET_HUGE = 12;
+constexpr uint32_t WAVE_GRAB = 8;  // wave-phase tails taken per queue access
 
-// work-queue header layout (u32 words at the start of the workspace)
-enum : int {
-    Q_CNT_HUGE = 0,
-    Q_CNT_GROUP = 1,
-    Q_CNT_WAVE = 2,
-    Q_PULL_HUGE = 3,
-    Q_PULL_GROUP = 4,
-    Q_PULL_WAVE = 5,
-    Q_WORDS = 16
-};
-// stats accumulators (u64) follow the queue header
+// work-queue header (u32 words at the start of the workspace):
+//   [0, 13)   tails per bucket
+//   [16, 29)  pull cursor per bucket
+//   [32, 45)  fill cursor per bucket (binning pass 2)
+enum : int { Q_CNT = 0, Q_PULL = 16, Q_FILL = 32, Q_WORDS = 64 };
 enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_WORDS = 4 };
 
 template <bool PLAIN>
@@ -70,7 +73,7 @@ struct View {
             e = v.t_end ? v.t_end[u] : v.t_beg[u + 1];
         }
     }
-    __device__ __forceinline__ uint32_t t_head(uint64_t i) const { return PLAIN ? v.sheads[i] : v.theads[i]; }
+    __device__ __forceinline__ const uint32_t* t_heads() const { return PLAIN ? v.sheads : v.theads; }
     __device__ __forceinline__ uint32_t tail_pv(uint32_t x) const {
         return (PLAIN || v.tail_id == nullptr) ? x : v.tail_id[x];
     }
@@ -81,53 +84,105 @@ struct View {
 
 __device__ __forceinline__ uint32_t ceil_log2(uint64_t x) { return x <= 1 ? 0 : 64 - __clzll(x - 1); }
 
-// ---------------------------------------------------------------------------
-// binning: split tails into huge / group / wave lists
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void wave_append(uint32_t* list, uint32_t* cnt, bool pred, uint32_t item) {
-    uint64_t m = __ballot(pred);
-    if (m == 0) return;
-    uint32_t leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(cnt, (uint32_t)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (pred) list[base + __popcll(m & lanemask_lt())] = item;
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
 }
 
+// ---------------------------------------------------------------------------
+// binning: bucket tails by ceil(log2 |S(x)|), two passes (count, fill)
+// ---------------------------------------------------------------------------
 template <bool PLAIN>
-__global__ void __launch_bounds__(256) k_tc_bin(View<PLAIN> V, TcParams prm, uint32_t* __restrict__ q,
-                                                uint32_t* __restrict__ l_huge, uint32_t* __restrict__ l_group,
-                                                uint32_t* __restrict__ l_wave, uint64_t* __restrict__ stats) {
-    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    bool in = x < V.v.X;
-    uint64_t sb = 0, se = 0, rb = 0, re = 0;
-    if (in) {
-        V.s_range(x, sb, se);
-        V.r_range(x, rb, re);
-    }
-    uint64_t s = se - sb, r = re - rb;
-    bool active = in && s > 0 && r > 0 && !(PLAIN && s < 2);
-    // a plain tail with |N+(x)| == 1 emits nothing but still counts one
-    // intersection in the reference's counters (cetric_edge_iterator.h:279-281)
+__device__ __forceinline__ int tail_bucket(const View<PLAIN>& V, const TcParams& prm, uint32_t x, uint64_t& s,
+                                           uint64_t& r) {
+    uint64_t sb, se, rb, re;
+    V.s_range(x, sb, se);
+    V.r_range(x, rb, re);
+    s = se - sb;
+    r = re - rb;
+    if (s == 0 || r == 0 || (PLAIN && s < 2)) return -1;
+    if (s > prm.group_max_deg) return BUCKET_HUGE;
+    return (int)ceil_log2(s);  // 0..11
+}
+
+// Grid-stride over tails; per-block bucket counts in LDS, one global atomic
+// per bucket (and per stat) per block.
+template <bool PLAIN>
+__global__ void __launch_bounds__(256) k_tc_bin_count(View<PLAIN> V, TcParams prm, uint32_t* __restrict__ q,
+                                                      uint64_t* __restrict__ stats) {
+    __shared__ uint32_t cnt[NBUCKET];
+    __shared__ uint64_t red[2][4];
+    if (threadIdx.x < NBUCKET) cnt[threadIdx.x] = 0;
+    __syncthreads();
     uint64_t wedge = 0, isz = 0;
-    if (PLAIN && in && s == 1) {
-        uint64_t tb, te;
-        V.t_range(V.r_head(rb), tb, te);
-        wedge = te - tb;
-        isz = 1 + (te - tb);
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < V.v.X; x += gridDim.x * blockDim.x) {
+        uint64_t s, r;
+        int b = tail_bucket(V, prm, x, s, r);
+        if (b >= 0) atomicAdd(&cnt[b], 1u);
+        // a plain tail with |N+(x)| == 1 emits nothing but still counts one
+        // intersection in the reference's counters (cetric_edge_iterator.h:279-281)
+        if (PLAIN && s == 1) {
+            uint64_t sb, se, tb, te;
+            V.s_range(x, sb, se);
+            V.t_range(V.r_head(sb), tb, te);
+            wedge += te - tb;
+            isz += 1 + (te - tb);
+        }
     }
     wedge = wave_sum(wedge);
     isz = wave_sum(isz);
-    if (lane_id() == 0 && (wedge | isz)) {
-        atomic_add_u64(&stats[ST_WEDGE], wedge);
-        atomic_add_u64(&stats[ST_ISZ], isz);
+    if (lane_id() == 0) {
+        red[0][threadIdx.x / KT_WAVE] = wedge;
+        red[1][threadIdx.x / KT_WAVE] = isz;
     }
-    bool huge = active && s > prm.group_max_deg;
-    bool group = active && !huge && s > prm.wave_max_deg;
-    bool wave = active && !huge && !group;
-    wave_append(l_huge, &q[Q_CNT_HUGE], huge, x);
-    wave_append(l_group, &q[Q_CNT_GROUP], group, x);
-    wave_append(l_wave, &q[Q_CNT_WAVE], wave, x);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        uint64_t c = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        if (a) atomic_add_u64(&stats[ST_WEDGE], a);
+        if (c) atomic_add_u64(&stats[ST_ISZ], c);
+    }
+    if (threadIdx.x < NBUCKET && cnt[threadIdx.x]) atomicAdd(&q[Q_CNT + threadIdx.x], cnt[threadIdx.x]);
+}
+
+// Bucket regions in one list: bucket b starts at sum of counts of b' < b.
+__device__ __forceinline__ uint32_t bucket_base(const uint32_t* q, int b) {
+    uint32_t s = 0;
+    for (int i = 0; i < b; ++i) s += q[Q_CNT + i];
+    return s;
+}
+
+// Each block owns a contiguous range of tails: it counts its tails per
+// bucket, reserves one range per bucket with a single global atomic, then
+// writes its tails into the reserved ranges.
+template <bool PLAIN>
+__global__ void __launch_bounds__(256) k_tc_bin_fill(View<PLAIN> V, TcParams prm, uint32_t* __restrict__ q,
+                                                     uint32_t* __restrict__ list) {
+    __shared__ uint32_t cnt[NBUCKET];
+    __shared__ uint32_t pos[NBUCKET];
+    const uint32_t X = V.v.X;
+    const uint32_t per = (X + gridDim.x - 1) / gridDim.x;
+    const uint32_t x0 = min(X, blockIdx.x * per), x1 = min(X, x0 + per);
+    if (threadIdx.x < NBUCKET) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t x = x0 + threadIdx.x; x < x1; x += blockDim.x) {
+        uint64_t s, r;
+        int b = tail_bucket(V, prm, x, s, r);
+        if (b >= 0) atomicAdd(&cnt[b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < NBUCKET) {
+        const uint32_t c = cnt[threadIdx.x];
+        pos[threadIdx.x] = c ? bucket_base(q, threadIdx.x) + atomicAdd(&q[Q_FILL + threadIdx.x], c) : 0;
+    }
+    __syncthreads();
+    for (uint32_t x = x0 + threadIdx.x; x < x1; x += blockDim.x) {
+        uint64_t s, r;
+        int b = tail_bucket(V, prm, x, s, r);
+        if (b >= 0) list[atomicAdd(&pos[b], 1u)] = x;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -136,9 +191,8 @@ __global__ void __launch_bounds__(256) k_tc_bin(View<PLAIN> V, TcParams prm, uin
 struct Lds {
     uint32_t keys[GROUP_SLOTS];
     uint32_t cnt[GROUP_SLOTS];
-    int marker[TC_WAVES][KT_WAVE];
-    uint32_t ucnt[TC_WAVES][KT_WAVE];
     uint64_t red[TC_WAVES];
+    uint32_t next_u;
     uint32_t bcast;
 };
 
@@ -150,151 +204,275 @@ __device__ __forceinline__ void team_sync() {
         wave_sync();
 }
 
-// Probe policy: LDS hash table (HASH) or binary search of the sorted S(x) in
-// global memory (huge tails).
-template <bool HASH>
-struct Probe {
-    uint32_t* keys;
-    uint32_t* cnt;
-    uint32_t logS;
-    uint32_t mask;
-    const uint32_t* s;  // sorted S(x) (global) for !HASH
-    uint64_t ns;
-    uint64_t* per_vertex;
+// Bucketed hash set in LDS: 4-slot buckets (16 bytes, read with one
+// ds_read_b128), filled left to right; a key lives in its home bucket
+// hash(w) or, when that was full at insertion, in a later bucket (linear
+// probing over buckets).  A lookup therefore ends at the first bucket that
+// holds w or has an empty last slot.
+__device__ __forceinline__ uint32_t bucket_of(uint32_t w, uint32_t logB) { return hash_slot(w, logB); }
 
-    // Returns true on hit and records the w-side count.
-    __device__ __forceinline__ bool probe(uint32_t w) const {
-        if (HASH) {
-            uint32_t h = hash_slot(w, logS);
-            while (true) {
-                uint32_t k = keys[h];
-                if (k == w) {
-                    atomicAdd(&cnt[h], 1u);
-                    return true;
+__device__ __forceinline__ void table_insert(uint32_t* keys, uint32_t logB, uint32_t bmask, uint32_t w) {
+    uint32_t b = bucket_of(w, logB);
+    while (true) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            uint32_t old = atomicCAS(&keys[4 * b + s], KT_EMPTY, w);
+            if (old == KT_EMPTY || old == w) return;
+        }
+        b = (b + 1) & bmask;
+    }
+}
+
+__device__ __forceinline__ uint32_t match4(const uint4& k, uint32_t w) {
+    return (k.x == w ? 1u : 0u) | (k.y == w ? 2u : 0u) | (k.z == w ? 4u : 0u) | (k.w == w ? 8u : 0u);
+}
+
+// Probes up to 4 ids of this lane (valid bits in vmask); returns the number of
+// hits and bumps each hit slot's w-counter.
+__device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, uint32_t logB, uint32_t bmask,
+                                           const uint32_t (&w)[4], uint32_t vmask) {
+    const uint4* kb = reinterpret_cast<const uint4*>(keys);
+    uint32_t bk[4];
+    uint4 kk[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        bk[e] = bucket_of(w[e], logB);
+        kk[e] = kb[bk[e]];
+    }
+    uint32_t hits = 0, cont = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t m = match4(kk[e], w[e]);
+        const bool valid = (vmask >> e) & 1u;
+        if (valid && m) {
+            atomicAdd(&cnt[4 * bk[e] + __builtin_ctz(m)], 1u);
+            ++hits;
+        }
+        if (valid && !m && kk[e].w != KT_EMPTY) cont |= 1u << e;
+    }
+    // rare: the home bucket was full and did not hold the id
+    while (__ballot(cont != 0)) {
+        if (cont) {
+            const int e = __builtin_ctz(cont);
+            uint32_t wb = w[0];
+            uint32_t b = bk[0];
+#pragma unroll
+            for (int q = 1; q < 4; ++q)
+                if (e == q) {
+                    wb = w[q];
+                    b = bk[q];
                 }
-                if (k == KT_EMPTY) return false;
-                h = (h + 1) & mask;
+            b = (b + 1) & bmask;
+            while (true) {
+                const uint4 k = kb[b];
+                const uint32_t m = match4(k, wb);
+                if (m) {
+                    atomicAdd(&cnt[4 * b + __builtin_ctz(m)], 1u);
+                    ++hits;
+                    break;
+                }
+                if (k.w == KT_EMPTY) break;
+                b = (b + 1) & bmask;
             }
-        } else {
-            uint64_t lo = 0, hi = ns;
-            while (lo < hi) {
-                uint64_t mid = (lo + hi) >> 1;
-                if (s[mid] < w)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            if (lo < ns && s[lo] == w) {
-                if (per_vertex) atomic_add_u64(&per_vertex[w], 1);
-                return true;
-            }
-            return false;
+            cont &= cont - 1;
         }
     }
+    return hits;
+}
+
+// Binary-search probe of the sorted S(x) in global memory (huge tails).
+__device__ __forceinline__ uint32_t probe_bsearch(const uint32_t* __restrict__ s, uint64_t ns, uint64_t* pv,
+                                                  uint32_t w) {
+    uint64_t lo = 0, hi = ns;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (s[mid] < w)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo < ns && s[lo] == w) {
+        if (pv) atomic_add_u64(&pv[w], 1);
+        return 1;
+    }
+    return 0;
+}
+
+__device__ __forceinline__ uint32_t readlane32(uint32_t x, int j) { return __builtin_amdgcn_readlane(x, j); }
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int j) {
+    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, j);
+    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), j);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+struct ProbeCtx {
+    uint32_t* keys;
+    uint32_t* cnt;
+    uint32_t logB;   // log2(buckets)
+    uint32_t bmask;  // buckets - 1
+    const uint32_t* s;  // sorted S(x) (binary-search probes)
+    uint64_t ns;
+    uint64_t* pv;
 };
 
-// Process tail x with a team of NW waves.  Returns (in lane 0 of wave 0 of
-// the team) nothing; accumulates per-lane hits/stats into the references.
+// Streams the T lists of up to 64 neighbours R[k0, k0+cnt) through the probe.
+// Lane j first loads u_j and the bounds of T(u_j) (one coalesced load plus two
+// gathers for the whole chunk); the wave then walks the non-empty lists in
+// order, four ids per lane per step, prefetching the next block -- of the
+// same list or the next one -- before probing the current block.  Hits per u
+// are wave-summed and flushed once; returns the chunk's hits (wave-uniform).
+template <bool HASH, bool PLAIN>
+__device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0, uint32_t cnt, const ProbeCtx& P,
+                                               uint64_t ns, uint64_t& st_wedge, uint64_t& st_isz) {
+    const uint32_t lane = lane_id();
+    uint32_t uj = 0;
+    uint64_t tbj = 0, tej = 0;
+    if (lane < cnt) {
+        uj = V.r_head(k0 + lane);
+        V.t_range(uj, tbj, tej);
+    }
+    {
+        const uint64_t cw = uniform64(wave_sum(tej - tbj));
+        st_wedge += cw;
+        st_isz += cw + (uint64_t)cnt * ns;
+    }
+    uint64_t pending = __ballot(lane < cnt && tej > tbj);
+    if (pending == 0) return 0;
+    const uint32_t* __restrict__ T = V.t_heads();
+
+    int j = __ffsll((unsigned long long)pending) - 1;
+    pending &= pending - 1;
+    uint32_t u = readlane32(uj, j);
+    uint64_t tb = readlane64(tbj, j), te = readlane64(tej, j);
+    uint64_t wbase = tb & ~3ull;  // wave block base (uniform)
+    uint4 q = make_uint4(KT_EMPTY, KT_EMPTY, KT_EMPTY, KT_EMPTY);
+    if (wbase + 4 * lane < te) q = *reinterpret_cast<const uint4*>(T + wbase + 4 * lane);
+    uint32_t hits = 0;
+    uint64_t total = 0;
+    while (true) {
+        // next block: same list, or the first block of the next non-empty list
+        uint64_t nbase = wbase + 4 * KT_WAVE;
+        uint32_t nu = u;
+        uint64_t ntb = tb, nte = te;
+        bool more = true;
+        bool switch_u = false;
+        if (nbase >= te) {
+            switch_u = true;
+            if (pending) {
+                const int nj = __ffsll((unsigned long long)pending) - 1;
+                pending &= pending - 1;
+                nu = readlane32(uj, nj);
+                ntb = readlane64(tbj, nj);
+                nte = readlane64(tej, nj);
+                nbase = ntb & ~3ull;
+            } else {
+                more = false;
+            }
+        }
+        uint4 qn = make_uint4(KT_EMPTY, KT_EMPTY, KT_EMPTY, KT_EMPTY);
+        if (more && nbase + 4 * lane < nte) qn = *reinterpret_cast<const uint4*>(T + nbase + 4 * lane);
+        // probe the current block: element e of this lane sits at wbase + 4*lane + e
+        const uint32_t len = (uint32_t)(te - tb);
+        const uint32_t rel0 = (uint32_t)(wbase - tb) + 4 * lane;  // may wrap below zero (-3..-1)
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        uint32_t vmask = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) vmask |= (rel0 + e < len) ? (1u << e) : 0u;
+        if (HASH) {
+            hits += probe4(P.keys, P.cnt, P.logB, P.bmask, w, vmask);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if ((vmask >> e) & 1u) hits += probe_bsearch(P.s, P.ns, P.pv, w[e]);
+        }
+        if (switch_u) {
+            const uint32_t h = wave_sum(hits);
+            if (h) {
+                total += h;
+                if (lane == 0 && P.pv) atomic_add_u64(&P.pv[u], h);
+            }
+            hits = 0;
+        }
+        if (!more) break;
+        u = nu;
+        tb = ntb;
+        te = nte;
+        wbase = nbase;
+        q = qn;
+    }
+    return total;
+}
+
+// Process tail x with a team of NW waves (NW == 1: this wave alone).
 template <int NW, bool HASH, bool PLAIN>
 __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, Lds& L, uint32_t* keys,
-                                             uint32_t* cnt, uint32_t cap_slots, const TcOut& out,
+                                             uint32_t* cnt, uint32_t cap_slots, uint32_t chunk, const TcOut& out,
                                              uint64_t& my_total, uint64_t& st_wedge, uint64_t& st_isz) {
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / KT_WAVE;
-    const unsigned twave = NW > 1 ? wave : 0;  // wave index within the team
+    const unsigned twave = NW > 1 ? wave : 0;
     const unsigned tid = twave * KT_WAVE + lane;
-    int* marker = L.marker[wave];
-    uint32_t* ucnt = L.ucnt[wave];
 
     uint64_t sb, se, rb, re;
     V.s_range(x, sb, se);
     V.r_range(x, rb, re);
+    sb = uniform64(sb);
+    se = uniform64(se);
+    rb = uniform64(rb);
+    re = uniform64(re);
     const uint64_t ns = se - sb, nr = re - rb;
 
-    Probe<HASH> P;
-    P.per_vertex = out.per_vertex;
-    P.s = V.v.sheads + sb;
-    P.ns = ns;
+    ProbeCtx P;
     P.keys = keys;
     P.cnt = cnt;
-    P.logS = 0;
-    P.mask = 0;
+    P.logB = 0;
+    P.bmask = 0;
+    P.s = V.v.sheads + sb;
+    P.ns = ns;
+    P.pv = out.per_vertex;
     uint32_t slots = 0;
     if (HASH) {
-        uint32_t logS = ceil_log2(2 * ns);
+        uint32_t logS = ceil_log2(4 * ns);
         if (logS < MIN_LOGS) logS = MIN_LOGS;
         slots = 1u << logS;
-        if (slots > cap_slots) slots = cap_slots;  // never happens: bins respect the cap
-        P.logS = logS;
-        P.mask = slots - 1;
+        if (slots > cap_slots) {  // bins guarantee cap_slots >= 2 * ns
+            slots = cap_slots;
+            logS = ceil_log2(cap_slots);
+        }
+        P.logB = logS - 2;
+        P.bmask = (slots >> 2) - 1;
         for (uint32_t i = tid; i < slots; i += NW * KT_WAVE) {
             keys[i] = KT_EMPTY;
             cnt[i] = 0;
         }
         team_sync<NW>();
-        for (uint64_t i = tid; i < ns; i += NW * KT_WAVE) {
-            uint32_t w = V.v.sheads[sb + i];
-            uint32_t h = hash_slot(w, logS);
-            while (true) {
-                uint32_t old = atomicCAS(&keys[h], KT_EMPTY, w);
-                if (old == KT_EMPTY || old == w) break;
-                h = (h + 1) & P.mask;
-            }
+        for (uint64_t i = tid; i < ns; i += NW * KT_WAVE) table_insert(keys, P.logB, P.bmask, V.v.sheads[sb + i]);
+    }
+    if (NW > 1 && tid == 0) L.next_u = NW * chunk;
+    team_sync<NW>();
+
+    uint64_t tail_hits = 0;
+    uint64_t k = (uint64_t)twave * chunk;
+    while (k < nr) {
+        const uint32_t c = (uint32_t)min<uint64_t>(chunk, nr - k);
+        tail_hits += scan_chunk<HASH, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz);
+        if (NW > 1) {
+            uint32_t nk = 0;
+            if (lane == 0) nk = atomicAdd(&L.next_u, chunk);
+            k = uniform(nk);
+        } else {
+            k += chunk;
         }
-        team_sync<NW>();
     }
 
-    uint64_t hits = 0;
-    for (uint64_t c = (uint64_t)twave * KT_WAVE; c < nr; c += (uint64_t)NW * KT_WAVE) {
-        const uint64_t idx = rb + c + lane;
-        const bool valid = c + lane < nr;
-        uint32_t u = 0;
-        uint64_t tb = 0, te = 0;
-        if (valid) {
-            u = V.r_head(idx);
-            V.t_range(u, tb, te);
-        }
-        const uint64_t du = te - tb;
-        st_wedge += du;
-        st_isz += valid ? ns + du : 0;
-        const uint64_t incl = wave_incl_sum(du);
-        const uint64_t start = incl - du;
-        const uint64_t total = __shfl(incl, KT_WAVE - 1, 64);
-        ucnt[lane] = 0;
-        for (uint64_t t = 0; t < total; t += KT_WAVE) {
-            marker[lane] = -1;
-            wave_sync();
-            if (du > 0 && start >= t && start < t + KT_WAVE) marker[start - t] = (int)lane;
-            wave_sync();
-            const int mk = marker[lane];
-            const uint64_t ball = __ballot(incl > t);
-            const int owner_t = __ffsll((unsigned long long)ball) - 1;
-            const int own = max(owner_t, wave_incl_max(mk));
-            const uint64_t p = t + lane;
-            const uint64_t own_tb = __shfl(tb, own, 64);
-            const uint64_t own_start = __shfl(start, own, 64);
-            if (p < total) {
-                const uint32_t w = V.t_head(own_tb + (p - own_start));
-                if (P.probe(w)) atomicAdd(&ucnt[own], 1u);
-            }
-            wave_sync();
-        }
-        const uint32_t cu = ucnt[lane];
-        if (valid && cu) {
-            hits += cu;
-            if (out.per_vertex) atomic_add_u64(&out.per_vertex[u], cu);
-        }
-        wave_sync();
-    }
-
-    // team reduction of hits for the tail's own count and the graph total
-    uint64_t h = wave_sum(hits);
+    // team reduction of the tail's hits
+    uint64_t hsum = tail_hits;  // wave-uniform
     if (NW > 1) {
-        if (lane == 0) L.red[wave] = h;
+        if (lane == 0) L.red[wave] = hsum;
         __syncthreads();
-        h = 0;
+        hsum = 0;
 #pragma unroll
-        for (int i = 0; i < NW; ++i) h += L.red[i];
+        for (int i = 0; i < NW; ++i) hsum += L.red[i];
     }
     if (HASH) {
         team_sync<NW>();
@@ -305,72 +483,91 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
             }
         }
     }
-    if (tid == 0 && h) {
-        if (out.per_vertex) atomic_add_u64(&out.per_vertex[V.tail_pv(x)], h);
+    if (tid == 0 && hsum) {
+        if (out.per_vertex) atomic_add_u64(&out.per_vertex[V.tail_pv(x)], hsum);
         if (V.v.tail_graph)
-            atomic_add_u64(&out.totals[V.tail_graph(x)], h);
+            atomic_add_u64(&out.totals[V.tail_graph(x)], hsum);
         else
-            my_total += h;
+            my_total += hsum;
     }
-    team_sync<NW>();  // table / marker reuse by the next tail
+    team_sync<NW>();  // table reuse by the next tail
 }
 
 template <bool PLAIN>
-__global__ void __launch_bounds__(TC_THREADS) k_tc_main(View<PLAIN> V, TcOut out, uint32_t* __restrict__ q,
-                                                         const uint32_t* __restrict__ l_huge,
-                                                         const uint32_t* __restrict__ l_group,
-                                                         const uint32_t* __restrict__ l_wave,
+__global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) k_tc_main(View<PLAIN> V, TcOut out, TcParams prm,
+                                                         uint32_t* __restrict__ q,
+                                                         const uint32_t* __restrict__ list,
                                                          uint64_t* __restrict__ stats) {
     __shared__ Lds L;
     const unsigned wave = threadIdx.x / KT_WAVE;
     const unsigned lane = lane_id();
     uint64_t my_total = 0, st_wedge = 0, st_isz = 0;
+    const int wave_bucket_max = (int)ceil_log2(prm.wave_max_deg);
 
-    // phase 0: huge tails, 4-wave team, binary search probes
-    {
-        const uint32_t cnt = q[Q_CNT_HUGE];
+    // team phase: the workgroup-bin buckets from the largest down
+    for (int b = BUCKET_HUGE - 1; b > wave_bucket_max; --b) {
+        const uint32_t cnt = q[Q_CNT + b];
+        if (cnt == 0) continue;
+        const uint32_t base = bucket_base(q, b);
         while (true) {
-            if (threadIdx.x == 0) L.bcast = atomicAdd(&q[Q_PULL_HUGE], 1u);
+            if (threadIdx.x == 0) L.bcast = atomicAdd(&q[Q_PULL + b], 1u);
             __syncthreads();
             const uint32_t i = L.bcast;
             __syncthreads();
             if (i >= cnt) break;
-            team_process<TC_WAVES, false, PLAIN>(V, l_huge[i], L, nullptr, nullptr, 0, out, my_total, st_wedge,
-                                                 st_isz);
+            team_process<TC_WAVES, true, PLAIN>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.team_chunk,
+                                                out, my_total, st_wedge, st_isz);
         }
     }
-    // phase 1: group tails, 4-wave team, shared LDS table
+    // wave phase: independent waves with private tables, largest buckets first
     {
-        const uint32_t cnt = q[Q_CNT_GROUP];
-        while (true) {
-            if (threadIdx.x == 0) L.bcast = atomicAdd(&q[Q_PULL_GROUP], 1u);
-            __syncthreads();
-            const uint32_t i = L.bcast;
-            __syncthreads();
-            if (i >= cnt) break;
-            team_process<TC_WAVES, true, PLAIN>(V, l_group[i], L, L.keys, L.cnt, GROUP_SLOTS, out, my_total,
-                                                st_wedge, st_isz);
-        }
-    }
-    // phase 2: wave tails, independent waves with private LDS tables
-    {
-        const uint32_t cnt = q[Q_CNT_WAVE];
         uint32_t* keys = L.keys + wave * WAVE_SLOTS;
         uint32_t* tcnt = L.cnt + wave * WAVE_SLOTS;
-        while (true) {
-            uint32_t i = 0;
-            if (lane == 0) i = atomicAdd(&q[Q_PULL_WAVE], 1u);
-            i = __shfl(i, 0, 64);
-            if (i >= cnt) break;
-            team_process<1, true, PLAIN>(V, l_wave[i], L, keys, tcnt, WAVE_SLOTS, out, my_total, st_wedge,
-                                         st_isz);
+        for (int b = wave_bucket_max; b >= 0; --b) {
+            const uint32_t cnt = q[Q_CNT + b];
+            if (cnt == 0) continue;
+            const uint32_t base = bucket_base(q, b);
+            while (true) {
+                uint32_t i = 0;
+                if (lane == 0) i = atomicAdd(&q[Q_PULL + b], WAVE_GRAB);
+                i = uniform(i);
+                if (i >= cnt) break;
+                const uint32_t e = min(cnt, i + WAVE_GRAB);
+                for (; i < e; ++i)
+                    team_process<1, true, PLAIN>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, KT_WAVE, out,
+                                                 my_total, st_wedge, st_isz);
+            }
         }
     }
-    // flush per-lane accumulators
-    my_total = wave_sum(my_total);
-    st_wedge = wave_sum(st_wedge);
-    st_isz = wave_sum(st_isz);
+    // flush per-wave accumulators (wave-uniform values)
     if (lane == 0) {
+        if (my_total) atomic_add_u64(&out.totals[0], my_total);
+        if (st_wedge) atomic_add_u64(&stats[ST_WEDGE], st_wedge);
+        if (st_isz) atomic_add_u64(&stats[ST_ISZ], st_isz);
+    }
+}
+
+// Tails whose S(x) exceeds the LDS table: 8-wave teams, binary-search probes
+// of the sorted S(x) in global memory.
+template <bool PLAIN>
+__global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out, TcParams prm,
+                                                         uint32_t* __restrict__ q,
+                                                         const uint32_t* __restrict__ list,
+                                                         uint64_t* __restrict__ stats) {
+    __shared__ Lds L;
+    uint64_t my_total = 0, st_wedge = 0, st_isz = 0;
+    const uint32_t cnt = q[Q_CNT + BUCKET_HUGE];
+    const uint32_t base = bucket_base(q, BUCKET_HUGE);
+    while (true) {
+        if (threadIdx.x == 0) L.bcast = atomicAdd(&q[Q_PULL + BUCKET_HUGE], 1u);
+        __syncthreads();
+        const uint32_t i = L.bcast;
+        __syncthreads();
+        if (i >= cnt) break;
+        team_process<TC_WAVES, false, PLAIN>(V, list[base + i], L, nullptr, nullptr, 0, prm.team_chunk, out,
+                                             my_total, st_wedge, st_isz);
+    }
+    if (lane_id() == 0) {
         if (my_total) atomic_add_u64(&out.totals[0], my_total);
         if (st_wedge) atomic_add_u64(&stats[ST_WEDGE], st_wedge);
         if (st_isz) atomic_add_u64(&stats[ST_ISZ], st_isz);
@@ -381,8 +578,8 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_main(View<PLAIN> V, TcOut out
 // host side
 // ---------------------------------------------------------------------------
 uint64_t tc_ws_words(uint32_t X) {
-    // queue header (u32) + stats (u64) + three u32 lists of X
-    return Q_WORDS / 2 + ST_WORDS + (3ull * X + 1) / 2 + 1;
+    // queue header (u32) + stats (u64) + one u32 list of X
+    return Q_WORDS / 2 + ST_WORDS + (uint64_t(X) + 1) / 2 + 1;
 }
 
 uint64_t* tc_stats_ptr(uint64_t* ws) { return ws + Q_WORDS / 2; }
@@ -394,18 +591,25 @@ static void launch_impl(const TcView& view, const TcOut& out, const TcParams& pr
     V.v = view;
     uint32_t* q = reinterpret_cast<uint32_t*>(ws);
     uint64_t* stats = ws + Q_WORDS / 2;
-    uint32_t* lists = reinterpret_cast<uint32_t*>(stats + ST_WORDS);
-    uint32_t* l_huge = lists;
-    uint32_t* l_group = lists + view.X;
-    uint32_t* l_wave = lists + 2ull * view.X;
+    uint32_t* list = reinterpret_cast<uint32_t*>(stats + ST_WORDS);
     fill_u64(ws, Q_WORDS / 2 + ST_WORDS, 0, st);
     TcParams p = prm;
     if (p.wave_max_deg > WAVE_SLOTS / 2) p.wave_max_deg = WAVE_SLOTS / 2;
     if (p.group_max_deg > GROUP_SLOTS / 2) p.group_max_deg = GROUP_SLOTS / 2;
-    if (view.X > 0) k_tc_bin<PLAIN><<<(view.X + 255) / 256, 256, 0, st>>>(V, p, q, l_huge, l_group, l_wave, stats);
+    // bucket edges are powers of two: round the wave threshold up to one
+    uint32_t wm = 1;
+    while (wm < p.wave_max_deg) wm <<= 1;
+    p.wave_max_deg = wm;
+    if (view.X > 0) {
+        const unsigned g = std::min<unsigned>((view.X + 255) / 256, (unsigned)num_cus * 8);
+        k_tc_bin_count<PLAIN><<<g, 256, 0, st>>>(V, p, q, stats);
+        k_tc_bin_fill<PLAIN><<<g, 256, 0, st>>>(V, p, q, list);
+    }
     unsigned grid = (unsigned)num_cus * (p.grid_per_cu ? p.grid_per_cu : 4);
+    // huge tails are rare: a small grid; it exits at once when there are none
+    k_tc_huge<PLAIN><<<(unsigned)num_cus, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
     if (ev_begin) hipEventRecord(ev_begin, st);
-    k_tc_main<PLAIN><<<grid, TC_THREADS, 0, st>>>(V, out, q, l_huge, l_group, l_wave, stats);
+    k_tc_main<PLAIN><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
     if (ev_end) hipEventRecord(ev_end, st);
 }
 
