@@ -70,6 +70,58 @@ __device__ __forceinline__ uint32_t row_of_slot(const uint64_t* __restrict__ off
     return lo;
 }
 
+__device__ __forceinline__ uint32_t wave_incl_max_u32(uint32_t x) {
+    unsigned l = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (l >= (unsigned)d) x = max(x, y);
+    }
+    return x;
+}
+
+// Resolves the CSR row of every slot of the tile [tile0, tend) of a CSR with
+// n rows: afterwards marks[i] = row(tile0 + i) - lo for i < tend - tile0.
+// Rows starting strictly inside the tile mark their first slot (the largest
+// row wins when empty rows share a start), then an inclusive max-scan over
+// the THREADS * ITEMS marks fills the gaps.  Block-wide: every thread calls it.
+template <int THREADS, int ITEMS>
+__device__ __forceinline__ void tile_rows(const uint64_t* __restrict__ off, uint32_t n, uint64_t tile0, uint64_t tend,
+                                          uint32_t* marks, uint32_t* bnd, uint32_t* wmax, uint32_t& lo,
+                                          uint32_t& hi) {
+    constexpr int TILE = THREADS * ITEMS;
+    const unsigned tid = threadIdx.x;
+    if (tid == 0) {
+        bnd[0] = row_of_slot(off, 0, n - 1, tile0);
+        bnd[1] = row_of_slot(off, 0, n - 1, tend - 1);
+    }
+    for (int i = tid; i < TILE; i += THREADS) marks[i] = 0;
+    __syncthreads();
+    lo = bnd[0];
+    hi = bnd[1];
+    for (uint64_t r = (uint64_t)lo + 1 + tid; r <= hi; r += THREADS) {
+        uint64_t pos = off[r] - tile0;
+        if (pos < (uint64_t)TILE) atomicMax(&marks[pos], (uint32_t)(r - lo));
+    }
+    __syncthreads();
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) m = max(m, marks[tid * ITEMS + i]);
+    uint32_t incl = wave_incl_max_u32(m);
+    if (lane_id() == KT_WAVE - 1) wmax[tid / KT_WAVE] = incl;
+    __syncthreads();
+    uint32_t carry = __shfl_up(incl, 1, 64);
+    if (lane_id() == 0) carry = 0;
+    for (unsigned w = 0; w < tid / KT_WAVE; ++w) carry = max(carry, wmax[w]);
+    uint32_t run = carry;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        run = max(run, marks[tid * ITEMS + i]);
+        marks[tid * ITEMS + i] = run;
+    }
+    __syncthreads();
+}
+
 // Multiplicative (Fibonacci) hash into a 2^logS table.
 __device__ __forceinline__ uint32_t hash_slot(uint32_t w, uint32_t logS) {
     return (w * 2654435761u) >> (32 - logS);

@@ -51,7 +51,9 @@ struct TcParams {
 //   triangle {pv_tail(x), u, w}.
 struct TcView {
     uint32_t X;                 // number of tails
-    const uint64_t* s_off;      // X+1
+    const uint64_t* s_off;      // X+1 (used when s_beg == nullptr)
+    const uint64_t* s_beg;      // X (nullable): S(x) = sheads[s_beg[x] .. s_end[x])
+    const uint64_t* s_end;      // X
     const uint32_t* sheads;
     const uint64_t* r_beg;      // X (nullptr: R = S)
     const uint64_t* r_end;      // X

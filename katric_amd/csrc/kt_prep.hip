@@ -145,16 +145,6 @@ __device__ __forceinline__ bool keep_edge(uint32_t v, uint32_t dv, uint32_t u, u
     return dv < du || (dv == du && v < u);
 }
 
-__device__ __forceinline__ uint32_t wave_incl_max_u32(uint32_t x) {
-    unsigned l = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
-        if (l >= (unsigned)d) x = max(x, y);
-    }
-    return x;
-}
-
 __global__ void __launch_bounds__(OR_THREADS)
     k_orient_keep(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
                   const uint32_t* __restrict__ deg, uint64_t* __restrict__ keep_bits,
@@ -167,43 +157,11 @@ __global__ void __launch_bounds__(OR_THREADS)
     const uint64_t tile0 = (uint64_t)blockIdx.x * OR_TILE;
     const uint64_t tend = min(tile0 + OR_TILE, nnz);
     const unsigned tid = threadIdx.x;
-    if (tid == 0) {
-        bnd[0] = row_of_slot(off, 0, n - 1, tile0);
-        bnd[1] = row_of_slot(off, 0, n - 1, tend - 1);
-    }
-    for (int i = tid; i < OR_TILE; i += OR_THREADS) {
-        marks[i] = 0;
-        rowcnt[i] = 0;
-    }
-    __syncthreads();
-    const uint32_t lo = bnd[0], hi = bnd[1];
+    for (int i = tid; i < OR_TILE; i += OR_THREADS) rowcnt[i] = 0;
+    uint32_t lo, hi;
+    tile_rows<OR_THREADS, OR_ITEMS>(off, n, tile0, tend, marks, bnd, wmax, lo, hi);
     const uint64_t nrows = (uint64_t)hi - lo + 1;
     const bool lds_rows = nrows <= OR_TILE;
-    // rows starting strictly inside the tile mark their first slot
-    for (uint64_t r = (uint64_t)lo + 1 + tid; r <= hi; r += OR_THREADS) {
-        uint64_t pos = off[r] - tile0;
-        if (pos < OR_TILE) atomicMax(&marks[pos], (uint32_t)(r - lo));
-    }
-    __syncthreads();
-    // inclusive max-scan over the 4096 marks (16 consecutive per thread)
-    {
-        uint32_t m = 0;
-#pragma unroll
-        for (int i = 0; i < OR_ITEMS; ++i) m = max(m, marks[tid * OR_ITEMS + i]);
-        uint32_t incl = wave_incl_max_u32(m);
-        if (lane_id() == KT_WAVE - 1) wmax[tid / KT_WAVE] = incl;
-        __syncthreads();
-        uint32_t carry = __shfl_up(incl, 1, 64);
-        if (lane_id() == 0) carry = 0;
-        for (unsigned w = 0; w < tid / KT_WAVE; ++w) carry = max(carry, wmax[w]);
-        uint32_t run = carry;
-#pragma unroll
-        for (int i = 0; i < OR_ITEMS; ++i) {
-            run = max(run, marks[tid * OR_ITEMS + i]);
-            marks[tid * OR_ITEMS + i] = run;
-        }
-    }
-    __syncthreads();
     uint32_t flags = 0, cnt = 0;
 #pragma unroll 4
     for (int k = 0; k < OR_ITEMS; ++k) {

@@ -1,0 +1,492 @@
+// One rank's shard of CeTriC ("cetric-x") / DiTriC ("patric") on one GPU.
+//
+// Vertex space: a combined index c over the rank's owned vertices
+// [0, n_local) (c = id - from) followed by its ghosts [n_local, n_local +
+// n_ghost) in ascending id order.  Sorting any list by c therefore equals the
+// reference's node_ordering::id_outward(rank) (owned first, then by id,
+// distributed_graph.h:102-112), and the ghost part of a list is its suffix.
+//
+// Preprocessing (one radix sort):
+//   every owned slot (v, u) with c = map(u) contributes
+//     (v, c)  if (deg v, v) < (deg u, u)           -- N+(v), global degrees
+//     (c, v)  if not, c is a ghost and cetric-x     -- ghost_adj(c)
+//   (remove_in_edges_and_expand_ghosts, distributed_graph.h:1057-1169, with
+//   node_ordering::degree on exchanged ghost degrees, :113-146).  The sorted
+//   (tail << b | head) keys form a CSR L over the combined space.
+// Local phase (same intersection kernel as the plain path):
+//   cetric-x: every tail x (owned or ghost), S = R = L(x), T(u) = L(u)
+//             (run_local_sequential_with_ghosts, cetric_edge_iterator.h:300-388);
+//   patric:   owned tails, S = L(v), R = owned part of L(v), T(u) = L(u)
+//             (run_local_sequential with ghost heads skipped, :229-298).
+// Contracted / distributed phase: message i = [a, list]; its entries split
+// into owned receivers R and ghosts G (entries unknown to this rank can match
+// nothing).  cetric-x: S = G, T(u) = N+cut(u) = ghost suffix of L(u)
+// (contraction, distributed_graph.h:679-764, then intersect_from_message,
+// cetric_edge_iterator.h:1797-1889); patric: S = R ++ G, T(u) = L(u).  Every
+// hit w emits Triangle{a, u, w}.
+#include <algorithm>
+
+#include "../../include/katric_shard.h"
+#include "kt_ctx.h"
+#include "kt_device.h"
+
+namespace kt {
+
+constexpr int SH_THREADS = 256;
+constexpr int SH_ITEMS = 16;
+constexpr int SH_TILE = SH_THREADS * SH_ITEMS;
+
+struct ShardMap {
+    uint32_t from, to, nl, ng;
+    const uint32_t* ghost_ids;
+    __device__ __forceinline__ uint32_t map(uint32_t u) const {
+        if (u >= from && u < to) return u - from;
+        uint32_t lo = 0, hi = ng;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (ghost_ids[mid] < u)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return (lo < ng && ghost_ids[lo] == u) ? nl + lo : 0xffffffffu;
+    }
+    __device__ __forceinline__ uint32_t gid(uint32_t c) const { return c < nl ? from + c : ghost_ids[c - nl]; }
+};
+
+__global__ void k_shard_prepare(ShardMap M, uint64_t nnz, const uint64_t* __restrict__ off,
+                                const uint64_t* __restrict__ ghost_degree, uint32_t* __restrict__ deg,
+                                uint32_t* __restrict__ err) {
+    const uint64_t X = (uint64_t)M.nl + M.ng;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < X; c += (uint64_t)gridDim.x * blockDim.x) {
+        if (c == 0 && (off[0] != 0 || off[M.nl] != nnz)) atomicOr(err, KT_ERR_OFFSETS);
+        if (c < M.nl) {
+            uint64_t a = off[c], b = off[c + 1];
+            if (b < a || b - a > 0xffffffffull) {
+                atomicOr(err, KT_ERR_OFFSETS);
+                deg[c] = 0;
+            } else {
+                deg[c] = (uint32_t)(b - a);
+            }
+        } else {
+            const uint64_t g = c - M.nl;
+            const uint32_t id = M.ghost_ids[g];
+            if ((id >= M.from && id < M.to) || (g > 0 && M.ghost_ids[g - 1] >= id)) atomicOr(err, KT_ERR_GHOST);
+            uint64_t d = ghost_degree[g];
+            deg[c] = d > 0xffffffffull ? 0xffffffffu : (uint32_t)d;
+        }
+    }
+}
+
+__device__ __forceinline__ bool c_less(const ShardMap& M, const uint32_t* deg, uint32_t a, uint32_t b) {
+    const uint32_t da = deg[a], db = deg[b];
+    return da < db || (da == db && M.gid(a) < M.gid(b));
+}
+
+__device__ __forceinline__ void wave_push(uint64_t* keys, uint64_t* ctr, bool pred, uint64_t key) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+    uint64_t base = 0;
+    if (lane_id() == leader) base = atomic_add_u64(ctr, (uint64_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (pred) keys[base + __popcll(m & lanemask_lt())] = key;
+}
+
+// Owned slots -> (tail, head) keys of N+ lists and (cetric-x) ghost lists.
+__global__ void __launch_bounds__(SH_THREADS)
+    k_shard_entries(ShardMap M, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
+                    const uint32_t* __restrict__ deg, int bits, int ghost_lists, uint64_t* __restrict__ keys,
+                    uint64_t* __restrict__ nkeys, uint32_t* __restrict__ err) {
+    __shared__ uint32_t marks[SH_TILE];
+    __shared__ uint32_t wmax[SH_THREADS / KT_WAVE];
+    __shared__ uint32_t bnd[2];
+    const uint64_t tile0 = (uint64_t)blockIdx.x * SH_TILE;
+    const uint64_t tend = min(tile0 + SH_TILE, nnz);
+    uint32_t lo, hi;
+    tile_rows<SH_THREADS, SH_ITEMS>(off, M.nl, tile0, tend, marks, bnd, wmax, lo, hi);
+    uint32_t flags = 0;
+    for (int k = 0; k < SH_ITEMS; ++k) {
+        const unsigned i = k * SH_THREADS + threadIdx.x;
+        const uint64_t sl = tile0 + i;
+        bool out_e = false, in_e = false;
+        uint32_t v = 0, c = 0;
+        if (sl < tend) {
+            v = lo + marks[i];
+            const uint32_t u = heads[sl];
+            c = M.map(u);
+            if (c == 0xffffffffu) {
+                flags |= KT_ERR_GHOST;
+            } else if (c == v) {
+                flags |= KT_ERR_SELF_LOOP;
+            } else {
+                out_e = c_less(M, deg, v, c);
+                in_e = ghost_lists && !out_e && c >= M.nl;
+            }
+        }
+        wave_push(keys, nkeys, out_e || in_e,
+                  out_e ? (((uint64_t)v << bits) | c) : (((uint64_t)c << bits) | v));
+    }
+    if (flags) atomicOr(err, flags);
+}
+
+// Sorted keys -> CSR over X tails (offsets by boundary detection).
+__global__ void k_keys_csr(uint64_t M, uint32_t X, int bits, const uint64_t* __restrict__ keys,
+                           uint64_t* __restrict__ coff, uint32_t* __restrict__ cheads, uint32_t* __restrict__ err) {
+    const uint64_t mask = (1ull << bits) - 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= M; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = i < M ? (keys[i] >> bits) : X;
+        const int64_t tp = i == 0 ? -1 : (int64_t)(keys[i - 1] >> bits);
+        if (i < M) {
+            cheads[i] = (uint32_t)(keys[i] & mask);
+            if (i > 0 && keys[i - 1] == keys[i]) atomicOr(err, KT_ERR_DUPLICATE);
+        }
+        for (int64_t r = tp + 1; r <= (int64_t)t; ++r) coff[r] = i;
+    }
+}
+
+// First position of a ghost head (combined index >= nl) in each owned list.
+__global__ void k_cut_begin(uint32_t nl, const uint64_t* __restrict__ coff, const uint32_t* __restrict__ cheads,
+                            uint64_t* __restrict__ cut) {
+    uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nl) return;
+    uint64_t lo = coff[u], hi = coff[u + 1];
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (cheads[mid] < nl)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    cut[u] = lo;
+}
+
+// Per message: owned / ghost entry counts; sender -> combined index.
+__global__ void k_msg_count(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint32_t* __restrict__ src,
+                            const uint64_t* __restrict__ moff, const uint32_t* __restrict__ mheads,
+                            uint64_t* __restrict__ kept, uint32_t* __restrict__ nown, uint32_t* __restrict__ tail_id,
+                            uint32_t* __restrict__ err) {
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KT_WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / KT_WAVE;
+    for (uint32_t i = wid; i < nmsg; i += nw) {
+        const uint64_t b = moff[i], e = moff[i + 1];
+        uint32_t own = 0, gh = 0;
+        if (e < b || e > mtot) {
+            if (lane_id() == 0) atomicOr(err, KT_ERR_MESSAGE);
+        } else {
+            for (uint64_t p = b + lane_id(); p < e; p += KT_WAVE) {
+                const uint32_t c = M.map(mheads[p]);
+                own += (c < M.nl) ? 1u : 0u;
+                gh += (c != 0xffffffffu && c >= M.nl) ? 1u : 0u;
+                if (p + 1 < e && mheads[p + 1] <= mheads[p]) atomicOr(err, KT_ERR_MESSAGE);
+            }
+        }
+        own = wave_sum(own);
+        gh = wave_sum(gh);
+        if (lane_id() == 0) {
+            const uint32_t a = M.map(src[i]);
+            if (a == 0xffffffffu || a < M.nl) atomicOr(err, KT_ERR_MESSAGE);
+            tail_id[i] = (a == 0xffffffffu) ? 0u : a;
+            kept[i] = own + gh;
+            nown[i] = own;
+        }
+    }
+}
+
+// Per message: owned entries then ghost entries, each in message order.
+__global__ void k_msg_write(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint64_t* __restrict__ moff,
+                            const uint32_t* __restrict__ mheads, const uint64_t* __restrict__ coff_msg,
+                            const uint32_t* __restrict__ nown, uint32_t* __restrict__ out,
+                            uint64_t* __restrict__ split) {
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KT_WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / KT_WAVE;
+    for (uint32_t i = wid; i < nmsg; i += nw) {
+        const uint64_t b = moff[i], e = moff[i + 1];
+        uint64_t po = coff_msg[i], pg = coff_msg[i] + nown[i];
+        if (lane_id() == 0) split[i] = pg;
+        if (e < b || e > mtot) continue;  // flagged by k_msg_count; nothing was counted for it
+        for (uint64_t p0 = b; p0 < e; p0 += KT_WAVE) {
+            const uint64_t p = p0 + lane_id();
+            uint32_t c = 0xffffffffu;
+            if (p < e) c = M.map(mheads[p]);
+            const bool own = c < M.nl;
+            const bool gh = c != 0xffffffffu && c >= M.nl;
+            const uint64_t mo = __ballot(own), mg = __ballot(gh);
+            if (own) out[po + __popcll(mo & lanemask_lt())] = c;
+            if (gh) out[pg + __popcll(mg & lanemask_lt())] = c;
+            po += __popcll(mo);
+            pg += __popcll(mg);
+        }
+    }
+}
+
+}  // namespace kt
+
+using namespace kt;
+
+namespace {
+
+int bits_for(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b)) ++b;
+    return b;
+}
+
+struct ShardBufs {
+    uint64_t* small;
+    const uint64_t* off;
+    const uint32_t* heads;
+    const uint32_t* ghost_ids;
+    const uint64_t* ghost_degree;
+    const uint32_t* msg_src;
+    const uint64_t* msg_off;
+    const uint32_t* msg_heads;
+    uint32_t* deg;
+    uint64_t* keys;
+    uint64_t* alt;
+    uint64_t* radix_tmp;
+    uint64_t* coff;
+    uint32_t* cheads;
+    uint64_t* cut;
+    uint64_t* mkept;
+    uint64_t* moff;
+    uint32_t* mnown;
+    uint32_t* mtail;
+    uint64_t* msplit;
+    uint32_t* mheads2;
+    uint64_t* scan_tmp;
+    uint64_t* tc_ws;
+    uint64_t* pv;
+};
+
+uint64_t shard_layout(void* base, const kt_shard_in* in, uint64_t nnz, uint64_t mtot, bool host_in, bool pv_dev,
+                      ShardBufs& B) {
+    Carver cv{base, 0};
+    const uint64_t nloc = in->range_bounds[in->rank + 1] - in->range_bounds[in->rank];
+    const uint64_t X = nloc + in->n_ghost;
+    B.small = static_cast<uint64_t*>(cv.take(64 * 8));
+    if (host_in) {
+        B.off = static_cast<uint64_t*>(cv.take((nloc + 1) * 8));
+        B.heads = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(nnz, 1) * 4));
+        B.ghost_ids = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(in->n_ghost, 1) * 4));
+        B.ghost_degree = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(in->n_ghost, 1) * 8));
+        B.msg_src = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 4));
+        B.msg_off = static_cast<uint64_t*>(cv.take(((uint64_t)in->n_msg + 1) * 8));
+        B.msg_heads = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(mtot, 1) * 4));
+    }
+    B.deg = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(X, 1) * 4));
+    B.keys = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(nnz, 1) * 8));
+    B.alt = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(nnz, 1) * 8));
+    B.radix_tmp = static_cast<uint64_t*>(cv.take(radix_tmp_words(nnz) * 8));
+    B.coff = static_cast<uint64_t*>(cv.take((X + 1) * 8));
+    B.cheads = static_cast<uint32_t*>(cv.take((nnz + 4) * 4));
+    B.cut = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(nloc, 1) * 8));
+    B.mkept = static_cast<uint64_t*>(cv.take(((uint64_t)in->n_msg + 1) * 8));
+    B.moff = static_cast<uint64_t*>(cv.take(((uint64_t)in->n_msg + 1) * 8));
+    B.mnown = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 4));
+    B.mtail = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 4));
+    B.msplit = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 8));
+    B.mheads2 = static_cast<uint32_t*>(cv.take((mtot + 4) * 4));
+    B.scan_tmp = static_cast<uint64_t*>(cv.take(scan_tmp_words((uint64_t)in->n_msg + 1) * 8));
+    B.tc_ws = static_cast<uint64_t*>(cv.take(tc_ws_words((uint32_t)std::max<uint64_t>(X, in->n_msg)) * 8));
+    B.pv = pv_dev ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(X, 1) * 8)) : nullptr;
+    return cv.used;
+}
+
+}  // namespace
+
+extern "C" int kt_count_shard(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) {
+    if (!c) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(true);
+    if (rc) return rc;
+    if (!in || !out || !in->range_bounds || !in->offsets || in->num_ranks == 0 || in->rank >= in->num_ranks ||
+        in->algorithm > KT_SHARD_DITRIC)
+        return c->fail(KT_EINVAL, "bad shard arguments");
+    const uint32_t P = in->num_ranks;
+    const uint64_t* bd = in->range_bounds;
+    if (bd[0] != 0) return c->fail(KT_EINVAL, "range_bounds[0] != 0");
+    for (uint32_t r = 0; r < P; ++r)
+        if (bd[r + 1] < bd[r] || bd[r + 1] > 0xffffffffull) return c->fail(KT_EINVAL, "range bounds not ascending");
+    const uint32_t from = (uint32_t)bd[in->rank], to = (uint32_t)bd[in->rank + 1];
+    const uint32_t nl = to - from, ng = in->n_ghost;
+    const uint64_t X = (uint64_t)nl + ng;
+    if (X >= 0xffffffffull) return c->fail(KT_EINVAL, "shard too large");
+    out->local_triangles = out->type3_triangles = 0;
+    const bool host_in = !(in->flags & KT_DEVICE_INPUT);
+    const bool want_pv = out->per_vertex != nullptr;
+    const bool pv_dev = want_pv && (in->flags & KT_DEVICE_OUTPUT);
+    if (nl == 0 && ng == 0) return KT_OK;
+    // sizes
+    uint64_t nnz = 0, mtot = 0;
+    if (host_in) {
+        if (in->offsets[0] != 0) return c->fail(KT_EINVAL, "offsets[0] != 0");
+        nnz = in->offsets[nl];
+        mtot = in->n_msg ? in->msg_offsets[in->n_msg] : 0;
+        if (in->n_msg && in->msg_offsets[0] != 0) return c->fail(KT_EINVAL, "msg_offsets[0] != 0");
+    } else {
+        rc = c->hip_check(hipMemcpy(&nnz, in->offsets + nl, 8, hipMemcpyDeviceToHost), "D2H nnz");
+        if (rc) return rc;
+        if (in->n_msg) {
+            rc = c->hip_check(hipMemcpy(&mtot, in->msg_offsets + in->n_msg, 8, hipMemcpyDeviceToHost), "D2H mtot");
+            if (rc) return rc;
+        }
+    }
+    if ((nnz && !in->heads) || (ng && (!in->ghost_ids || !in->ghost_degree)) ||
+        (in->n_msg && (!in->msg_src || !in->msg_offsets || (mtot && !in->msg_heads))))
+        return c->fail(KT_EINVAL, "null shard array");
+    ShardBufs B;
+    const uint64_t need = shard_layout(nullptr, in, nnz, mtot, host_in, want_pv && !pv_dev, B);
+    rc = c->reserve(need);
+    if (rc) return rc;
+    shard_layout(c->arena, in, nnz, mtot, host_in, want_pv && !pv_dev, B);
+    hipStream_t st = c->stream;
+    hipEventRecord(c->ev[EV_START], st);
+    if (host_in) {
+        struct {
+            const void* dst;
+            const void* src;
+            uint64_t bytes;
+        } up[] = {{B.off, in->offsets, ((uint64_t)nl + 1) * 8},
+                  {B.heads, in->heads, nnz * 4},
+                  {B.ghost_ids, in->ghost_ids, (uint64_t)ng * 4},
+                  {B.ghost_degree, in->ghost_degree, (uint64_t)ng * 8},
+                  {B.msg_src, in->msg_src, (uint64_t)in->n_msg * 4},
+                  {B.msg_off, in->msg_offsets, in->n_msg ? ((uint64_t)in->n_msg + 1) * 8 : 0},
+                  {B.msg_heads, in->msg_heads, mtot * 4}};
+        for (auto& u : up) {
+            if (!u.bytes) continue;
+            rc = c->hip_check(hipMemcpyAsync(const_cast<void*>(u.dst), u.src, u.bytes, hipMemcpyHostToDevice, st),
+                              "H2D shard");
+            if (rc) return rc;
+        }
+    } else {
+        B.off = in->offsets;
+        B.heads = in->heads;
+        B.ghost_ids = in->ghost_ids;
+        B.ghost_degree = in->ghost_degree;
+        B.msg_src = in->msg_src;
+        B.msg_off = in->msg_offsets;
+        B.msg_heads = in->msg_heads;
+    }
+    hipEventRecord(c->ev[EV_H2D], st);
+    uint32_t* err = reinterpret_cast<uint32_t*>(B.small);
+    uint64_t* nkeys = B.small + 1;
+    fill_u64(B.small, 64, 0, st);
+    ShardMap M{from, to, nl, ng, B.ghost_ids};
+    const int bits = std::max(1, bits_for(X));  // X itself is never a valid index: keys stay below 2^(2 bits)
+    const bool cetric = in->algorithm == KT_SHARD_CETRIC;
+    k_shard_prepare<<<(unsigned)std::min<uint64_t>((X + 255) / 256, 8192), 256, 0, st>>>(M, nnz, B.off,
+                                                                                      B.ghost_degree, B.deg, err);
+    const uint64_t tiles = (nnz + SH_TILE - 1) / SH_TILE;
+    if (tiles && nl)
+        k_shard_entries<<<(unsigned)tiles, SH_THREADS, 0, st>>>(M, nnz, B.off, B.heads, B.deg, bits, cetric ? 1 : 0,
+                                                                B.keys, nkeys, err);
+    rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 16, hipMemcpyDeviceToHost, st), "D2H flags");
+    if (rc) return rc;
+    rc = c->hip_check(hipStreamSynchronize(st), "shard entries");
+    if (rc) return rc;
+    uint32_t flags = (uint32_t)c->host_small[0];
+    const uint64_t Mk = c->host_small[1];
+    if (flags & KT_ERR_MASK)
+        return c->fail(KT_EINVAL, "malformed shard:%s%s%s%s", (flags & KT_ERR_OFFSETS) ? " bad offsets" : "",
+                       (flags & KT_ERR_GHOST) ? " ghost table" : "", (flags & KT_ERR_SELF_LOOP) ? " self-loop" : "",
+                       (flags & KT_ERR_DUPLICATE) ? " duplicate" : "");
+    radix_sort_u64(B.keys, B.alt, Mk, 2 * bits, B.radix_tmp, st);
+    k_keys_csr<<<(unsigned)std::min<uint64_t>((Mk + 256) / 256, 16384), 256, 0, st>>>(Mk, (uint32_t)X, bits, B.keys,
+                                                                                     B.coff, B.cheads, err);
+    if (nl) k_cut_begin<<<(nl + 255) / 256, 256, 0, st>>>(nl, B.coff, B.cheads, B.cut);
+    // messages
+    if (in->n_msg) {
+        k_msg_count<<<(unsigned)std::min<uint64_t>(((uint64_t)in->n_msg * 64 + 255) / 256, 8192), 256, 0, st>>>(
+            M, in->n_msg, mtot, B.msg_src, B.msg_off, B.msg_heads, B.mkept, B.mnown, B.mtail, err);
+        scan_u64(B.mkept, B.moff, in->n_msg, B.scan_tmp, st);
+        k_msg_write<<<(unsigned)std::min<uint64_t>(((uint64_t)in->n_msg * 64 + 255) / 256, 8192), 256, 0, st>>>(
+            M, in->n_msg, mtot, B.msg_off, B.msg_heads, B.moff, B.mnown, B.mheads2, B.msplit);
+    }
+    rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st), "D2H flags");
+    if (rc) return rc;
+    rc = c->hip_check(hipStreamSynchronize(st), "shard preprocessing");
+    if (rc) return rc;
+    flags = (uint32_t)c->host_small[0];
+    if (flags & KT_ERR_MASK)
+        return c->fail(KT_EINVAL, "malformed shard:%s%s", (flags & KT_ERR_DUPLICATE) ? " duplicate edge" : "",
+                       (flags & KT_ERR_MESSAGE) ? " bad message" : "");
+    hipEventRecord(c->ev[EV_PREP], st);
+    uint64_t* pv = want_pv ? (pv_dev ? out->per_vertex : B.pv) : nullptr;
+    if (pv) fill_u64(pv, X, 0, st);
+    uint64_t* totals = B.small + 4;  // [4] local, [5] type3
+    uint64_t* tstats = tc_stats_ptr(B.tc_ws);
+    // local phase
+    {
+        TcView v{};
+        v.sheads = B.cheads;
+        v.s_off = B.coff;
+        if (cetric) {
+            v.X = (uint32_t)X;  // plain view over owned + ghost tails
+        } else {
+            v.X = nl;
+            v.r_beg = B.coff;
+            v.r_end = B.cut;
+            v.rheads = B.cheads;
+            v.t_beg = B.coff;
+            v.t_end = B.coff + 1;
+            v.theads = B.cheads;
+        }
+        TcOut o{pv, totals};
+        tc_launch(v, o, c->prm, B.tc_ws, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]);
+        rc = c->hip_check(hipMemcpyAsync(c->host_small + 8, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
+        if (rc) return rc;
+    }
+    // contracted / distributed phase
+    if (in->n_msg) {
+        TcView v{};
+        v.X = in->n_msg;
+        v.sheads = B.mheads2;
+        v.rheads = B.mheads2;
+        v.r_beg = B.moff;
+        v.r_end = B.msplit;
+        if (cetric) {
+            v.s_beg = B.msplit;
+            v.s_end = B.moff + 1;
+            v.t_beg = B.cut;
+        } else {
+            v.s_off = B.moff;
+            v.t_beg = B.coff;
+        }
+        v.t_end = B.coff + 1;
+        v.theads = B.cheads;
+        v.tail_id = B.mtail;
+        TcOut o{pv, totals + 1};
+        tc_launch(v, o, c->prm, B.tc_ws, c->num_cus, st, nullptr, nullptr);
+        rc = c->hip_check(hipMemcpyAsync(c->host_small + 10, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
+        if (rc) return rc;
+    }
+    hipEventRecord(c->ev[EV_COUNT], st);
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 4, totals, 16, hipMemcpyDeviceToHost, st), "D2H totals");
+    if (rc) return rc;
+    if (want_pv && !pv_dev) {
+        rc = c->hip_check(hipMemcpyAsync(out->per_vertex, pv, X * 8, hipMemcpyDeviceToHost, st), "D2H per-vertex");
+        if (rc) return rc;
+    }
+    hipEventRecord(c->ev[EV_END], st);
+    rc = c->hip_check(hipStreamSynchronize(st), "shard count");
+    if (rc) return rc;
+    out->local_triangles = c->host_small[4];
+    out->type3_triangles = in->n_msg ? c->host_small[5] : 0;
+    c->stats.local_triangles = out->local_triangles;
+    c->stats.type3_triangles = out->type3_triangles;
+    c->stats.wedge_checks = c->host_small[8] + (in->n_msg ? c->host_small[10] : 0);
+    c->stats.intersection_size_local = c->host_small[9];
+    c->stats.intersection_size_global = in->n_msg ? c->host_small[11] : 0;
+    c->stats.oriented_edges = Mk;
+    c->stats.vertices = X;
+    c->stats.input_slots = nnz;
+    c->timing.h2d_ms = c->elapsed(EV_START, EV_H2D);
+    c->timing.prep_ms = c->elapsed(EV_H2D, EV_PREP);
+    c->timing.count_ms = c->elapsed(EV_PREP, EV_COUNT);
+    c->timing.intersect_ms = c->elapsed(EV_TC0, EV_TC1);
+    c->timing.d2h_ms = c->elapsed(EV_COUNT, EV_END);
+    c->timing.total_ms = c->elapsed(EV_START, EV_END);
+    return KT_OK;
+}

@@ -50,8 +50,13 @@ template <bool PLAIN>
 struct View {
     TcView v;
     __device__ __forceinline__ void s_range(uint32_t x, uint64_t& b, uint64_t& e) const {
-        b = v.s_off[x];
-        e = v.s_off[x + 1];
+        if (PLAIN || v.s_beg == nullptr) {
+            b = v.s_off[x];
+            e = v.s_off[x + 1];
+        } else {
+            b = v.s_beg[x];
+            e = v.s_end[x];
+        }
     }
     __device__ __forceinline__ void r_range(uint32_t x, uint64_t& b, uint64_t& e) const {
         if (PLAIN || v.r_beg == nullptr) {
@@ -615,8 +620,8 @@ static void launch_impl(const TcView& view, const TcOut& out, const TcParams& pr
 
 void tc_launch(const TcView& view, const TcOut& out, const TcParams& prm, uint64_t* ws, int num_cus,
                hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
-    const bool plain = view.r_beg == nullptr && view.rheads == nullptr && view.t_beg == nullptr &&
-                       view.theads == nullptr && view.tail_id == nullptr;
+    const bool plain = view.s_beg == nullptr && view.r_beg == nullptr && view.rheads == nullptr &&
+                       view.t_beg == nullptr && view.theads == nullptr && view.tail_id == nullptr;
     if (plain)
         launch_impl<true>(view, out, prm, ws, num_cus, st, ev_begin, ev_end);
     else
