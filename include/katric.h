@@ -128,6 +128,20 @@ int kt_count(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const uint32_t* h
 int kt_orient(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
               uint64_t* out_offsets, uint32_t* out_heads);
 
+/* Batched count of many independent graphs given as one disjoint-union CSR:
+ * graph g owns the global ids [vertex_base[g], vertex_base[g+1]) (host array
+ * of num_graphs+1), offsets has vertex_base[num_graphs]+1 entries and heads
+ * hold global ids.  totals (host, num_graphs) receives each graph's count,
+ * per_vertex (nullable) the per-vertex counts over the union.  The device
+ * pipeline is recorded into a HIP graph on the first call for a batch shape
+ * and replayed on later calls with the same shape and buffers.  No reference
+ * counterpart (the reference counts one graph per run, apps/shmetric.cpp:138-142);
+ * per graph the result equals kt_count. */
+int kt_batch_count(kt_ctx* ctx, uint32_t num_graphs, const uint64_t* vertex_base, const uint64_t* offsets,
+                   const uint32_t* heads, uint32_t flags, uint64_t* totals, uint64_t* per_vertex);
+/* Number of replays of the context's captured batch pipeline. */
+uint64_t kt_batch_replays(const kt_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,28 @@ int kt_read_graphio_binary(const char* first_out_path, const char* head_path, kt
 int kt_write_graphio_binary(const char* first_out_path, const char* head_path, uint32_t n, const uint64_t* offsets,
                             const uint32_t* heads);
 
+/* A batch of independent graphs as one disjoint-union CSR: graph g owns the
+ * global ids [vertex_base[g], vertex_base[g+1]); heads hold global ids. */
+typedef struct {
+    uint32_t num_graphs;
+    uint64_t n_total;
+    uint64_t nnz_total;
+    uint64_t* vertex_base; /* num_graphs + 1 */
+    uint64_t* offsets;     /* n_total + 1 */
+    uint32_t* heads;       /* nnz_total */
+    uint8_t* kind;         /* per graph: 0 = G(n,m), 1 = R-MAT */
+} kt_host_batch;
+
+void kt_host_batch_free(kt_host_batch* b);
+
+/* Mixed batch: graph g draws from its own seeded stream n = floor(exp(U[ln
+ * n_min, ln n_max])), an average degree d ~ U[deg_min, deg_max] and its kind
+ * (R-MAT (a, b, c) with probability rmat_fraction, else G(n,m)); both kinds
+ * take n*d/2 endpoint pairs, R-MAT at scale ceil(log2 n) rejecting ids >= n,
+ * ids permuted.  Every graph is canonical (as above). */
+int kt_gen_batch(uint32_t num_graphs, uint64_t seed, uint32_t n_min, uint32_t n_max, double deg_min, double deg_max,
+                 double rmat_fraction, double a, double b, double c, kt_host_batch* out);
+
 /* FNV-1a 64 over n, offsets[0..n] and heads[0..offsets[n]). */
 uint64_t kt_csr_checksum(uint32_t n, const uint64_t* offsets, const uint32_t* heads);
 

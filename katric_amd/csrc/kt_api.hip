@@ -552,6 +552,7 @@ extern "C" int kt_synchronize(kt_ctx* c) {
 }
 
 void kt_ctx::destroy_extensions() {
+    kt_batch_release(this);
     for (auto& kv : buffers) (void)hipFree(kv.first);
     buffers.clear();
 }

@@ -61,6 +61,8 @@ struct kt_ctx {
     void destroy_extensions();
 };
 
+void kt_batch_release(kt_ctx* c);
+
 namespace kt {
 uint64_t* tc_stats_ptr(uint64_t* ws);
 // KATRIC_TRACE=1: one stderr line per host-side pipeline step

@@ -12,6 +12,7 @@
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -162,27 +163,12 @@ void sample_keys(uint64_t samples, int nt, S&& sample, std::vector<uint64_t>& ke
     });
 }
 
-}  // namespace
-
-extern "C" {
-
-void kt_host_graph_free(kt_host_graph* g) {
-    if (!g) return;
-    free(g->offsets);
-    free(g->heads);
-    g->offsets = nullptr;
-    g->heads = nullptr;
-    g->n = 0;
-    g->nnz = 0;
-}
-
-int kt_gen_gnm(uint32_t n, uint64_t samples, uint64_t seed, kt_host_graph* out) {
-    if (!out) return KT_GIO_EINVAL;
+// G(n, m) / R-MAT into a canonical CSR using nt threads.
+int gen_gnm_impl(uint32_t n, uint64_t samples, uint64_t seed, int nt, kt_host_graph* out) {
     memset(out, 0, sizeof(*out));
     if (n == 0) return KT_GIO_EINVAL;
     const uint64_t key = stream_key(seed, 0);
     std::vector<uint64_t> keys;
-    const int nt = num_threads();
     sample_keys(samples, nt,
                 [&](uint64_t i, uint32_t& u, uint32_t& v) {
                     uint64_t r = draw(key, i);
@@ -194,16 +180,14 @@ int kt_gen_gnm(uint32_t n, uint64_t samples, uint64_t seed, kt_host_graph* out) 
     return keys_to_csr(keys, n, nt, out);
 }
 
-int kt_gen_rmat(uint32_t scale, uint64_t samples, double a, double b, double c, uint64_t seed, uint32_t n_limit,
-                int permute, kt_host_graph* out) {
-    if (!out) return KT_GIO_EINVAL;
+int gen_rmat_impl(uint32_t scale, uint64_t samples, double a, double b, double c, uint64_t seed, uint32_t n_limit,
+                  int permute, int nt, kt_host_graph* out) {
     memset(out, 0, sizeof(*out));
     if (scale == 0 || scale > 32 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return KT_GIO_EINVAL;
     const uint64_t full = 1ull << scale;
     const uint64_t n64 = n_limit ? std::min<uint64_t>(n_limit, full) : full;
     if (n64 > 0xffffffffull) return KT_GIO_EINVAL;
     const uint32_t n = (uint32_t)n64;
-    const int nt = num_threads();
     // seeded Fisher-Yates permutation of the surviving ids
     std::vector<uint32_t> perm;
     if (permute) {
@@ -243,6 +227,129 @@ int kt_gen_rmat(uint32_t scale, uint64_t samples, double a, double b, double c, 
                 },
                 keys);
     return keys_to_csr(keys, n, nt, out);
+}
+
+}  // namespace
+
+extern "C" {
+
+void kt_host_graph_free(kt_host_graph* g) {
+    if (!g) return;
+    free(g->offsets);
+    free(g->heads);
+    g->offsets = nullptr;
+    g->heads = nullptr;
+    g->n = 0;
+    g->nnz = 0;
+}
+
+int kt_gen_gnm(uint32_t n, uint64_t samples, uint64_t seed, kt_host_graph* out) {
+    if (!out) return KT_GIO_EINVAL;
+    return gen_gnm_impl(n, samples, seed, num_threads(), out);
+}
+
+int kt_gen_rmat(uint32_t scale, uint64_t samples, double a, double b, double c, uint64_t seed, uint32_t n_limit,
+                int permute, kt_host_graph* out) {
+    if (!out) return KT_GIO_EINVAL;
+    return gen_rmat_impl(scale, samples, a, b, c, seed, n_limit, permute, num_threads(), out);
+}
+
+void kt_host_batch_free(kt_host_batch* b) {
+    if (!b) return;
+    free(b->vertex_base);
+    free(b->offsets);
+    free(b->heads);
+    free(b->kind);
+    memset(b, 0, sizeof(*b));
+}
+
+// Mixed batch: graph g draws, from its own SplitMix64 stream, n =
+// floor(exp(U[ln n_min, ln n_max])), an average degree d ~ U[deg_min, deg_max]
+// and its kind (R-MAT with probability rmat_fraction, else G(n,m)); both kinds
+// take n*d/2 endpoint pairs; R-MAT uses scale ceil(log2 n), rejects ids >= n
+// and permutes ids.  Output: one disjoint-union CSR with global ids.
+int kt_gen_batch(uint32_t num_graphs, uint64_t seed, uint32_t n_min, uint32_t n_max, double deg_min, double deg_max,
+                 double rmat_fraction, double a, double b, double c, kt_host_batch* out) {
+    if (!out) return KT_GIO_EINVAL;
+    memset(out, 0, sizeof(*out));
+    if (num_graphs == 0 || n_min < 2 || n_max < n_min || deg_min < 0 || deg_max < deg_min) return KT_GIO_EINVAL;
+    std::vector<kt_host_graph> gs(num_graphs);
+    std::vector<uint8_t> kind(num_graphs);
+    std::atomic<uint32_t> next{0};
+    std::atomic<int> fail{0};
+    const int nt = num_threads();
+    parallel_for(nt, [&](int, int) {
+        while (true) {
+            uint32_t g = next.fetch_add(1);
+            if (g >= num_graphs) break;
+            const uint64_t k = stream_key(seed, 1000 + g);
+            const double u0 = (double)(draw(k, 0) >> 11) * 0x1.0p-53;
+            const double u1 = (double)(draw(k, 1) >> 11) * 0x1.0p-53;
+            const double u2 = (double)(draw(k, 2) >> 11) * 0x1.0p-53;
+            const double ln0 = std::log((double)n_min), ln1 = std::log((double)n_max);
+            uint32_t n = (uint32_t)std::floor(std::exp(ln0 + u0 * (ln1 - ln0)));
+            n = std::max(n_min, std::min(n, n_max));
+            const double d = deg_min + u1 * (deg_max - deg_min);
+            const uint64_t samples = (uint64_t)std::llround(n * d / 2.0);
+            const uint64_t gseed = draw(k, 3);
+            int rc;
+            if (u2 < rmat_fraction) {
+                kind[g] = 1;
+                uint32_t scale = 1;
+                while ((1ull << scale) < n) ++scale;
+                rc = gen_rmat_impl(scale, samples, a, b, c, gseed, n, 1, 1, &gs[g]);
+            } else {
+                kind[g] = 0;
+                rc = gen_gnm_impl(n, samples, gseed, 1, &gs[g]);
+            }
+            if (rc) fail.store(rc);
+        }
+    });
+    if (fail.load()) {
+        for (auto& g : gs) kt_host_graph_free(&g);
+        return fail.load();
+    }
+    uint64_t N = 0, E = 0;
+    for (auto& g : gs) {
+        N += g.n;
+        E += g.nnz;
+    }
+    if (N >= 0xffffffffull) {
+        for (auto& g : gs) kt_host_graph_free(&g);
+        return KT_GIO_EINVAL;
+    }
+    out->num_graphs = num_graphs;
+    out->n_total = N;
+    out->nnz_total = E;
+    out->vertex_base = (uint64_t*)malloc(((size_t)num_graphs + 1) * 8);
+    out->offsets = (uint64_t*)malloc((N + 1) * 8);
+    out->heads = (uint32_t*)malloc((E ? E : 1) * 4);
+    out->kind = (uint8_t*)malloc(num_graphs);
+    if (!out->vertex_base || !out->offsets || !out->heads || !out->kind) {
+        for (auto& g : gs) kt_host_graph_free(&g);
+        kt_host_batch_free(out);
+        return KT_GIO_ENOMEM;
+    }
+    memcpy(out->kind, kind.data(), num_graphs);
+    std::vector<uint64_t> ebase(num_graphs + 1, 0);
+    out->vertex_base[0] = 0;
+    for (uint32_t g = 0; g < num_graphs; ++g) {
+        out->vertex_base[g + 1] = out->vertex_base[g] + gs[g].n;
+        ebase[g + 1] = ebase[g] + gs[g].nnz;
+    }
+    next = 0;
+    parallel_for(nt, [&](int, int) {
+        while (true) {
+            uint32_t g = next.fetch_add(1);
+            if (g >= num_graphs) break;
+            const uint64_t vb = out->vertex_base[g], eb = ebase[g];
+            for (uint32_t v = 0; v < gs[g].n; ++v) out->offsets[vb + v] = eb + gs[g].offsets[v];
+            for (uint64_t e = 0; e < gs[g].nnz; ++e) out->heads[eb + e] = (uint32_t)(vb + gs[g].heads[e]);
+            kt_host_graph_free(&gs[g]);
+        }
+    });
+    out->offsets[N] = E;
+    return KT_GIO_OK;
 }
 
 // METIS text: header "n m [fmt [ncon]]", then one 1-based adjacency line per
