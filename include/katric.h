@@ -103,6 +103,9 @@ int kt_buffer_alloc(kt_ctx* ctx, uint64_t bytes, void** device_ptr);
 int kt_buffer_free(kt_ctx* ctx, void* device_ptr);
 int kt_buffer_upload(kt_ctx* ctx, void* device_dst, const void* host_src, uint64_t bytes);
 int kt_buffer_download(kt_ctx* ctx, void* host_dst, const void* device_src, uint64_t bytes);
+/* Page-locked host memory (faster, asynchronous host <-> device copies). */
+int kt_host_alloc(uint64_t bytes, void** host_ptr);
+int kt_host_free(void* host_ptr);
 /* Waits for all work of the context's device. */
 int kt_synchronize(kt_ctx* ctx);
 

@@ -1,0 +1,204 @@
+// katric_amd C++ host layer: the reference's entry points for the local
+// counting path, over the C ABI of katric.h / katric_shard.h.
+//
+// Mirrors (same names and argument meaning; paths relative to the reference
+// root):
+//   cetric::run_shmem(DistributedGraph<>&, profiling::Statistics&, const Config&)
+//       include/cetric/counters/shared_memory_cetric.h:105-170
+//   cetric::profiling::Statistics counters used on this path
+//       include/cetric/statistics.h:193-203
+//   cetric::Config knobs of this path (intersection method, threads)
+//       include/cetric/config.h:140-197
+//   DistributedGraph construction from a (first_out, head) CSR, rank 0 of 1
+//       include/cetric/datastructures/distributed/distributed_graph.h:797-819
+// and adds the rank-shard and batch entry points.
+//
+// Differences, by design: the triangle callback (emit) is not exported --
+// every reference caller only counts (shared_memory_cetric.h:130-147), and
+// 10^9 callbacks cannot cross the device boundary; the strategy knobs of
+// Config are accepted but the device picks its own intersection strategy
+// (counts are identical for every strategy).  Errors raise katric::Error
+// (the reference aborts through KASSERT in debug builds and has no status).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../katric.h"
+#include "../katric_shard.h"
+
+namespace katric {
+
+class Error : public std::runtime_error {
+public:
+    Error(int status, const std::string& what) : std::runtime_error(what), status_(status) {}
+    int status() const { return status_; }
+
+private:
+    int status_;
+};
+
+enum class IntersectionMethod { merge, binary_search, binary, hybrid };  // config.h:129
+
+struct Config {  // the fields of cetric::Config read on this path
+    IntersectionMethod intersection_method = IntersectionMethod::merge;
+    size_t binary_intersection_cutoff = 1000;
+    double hybrid_cutoff_scale = 1.0;
+    size_t num_threads = 1;
+    int device = 0;
+    uint64_t hbm_budget_bytes = 0;
+};
+
+namespace profiling {
+struct LocalStatistics {
+    uint64_t wedge_checks = 0;
+    uint64_t intersection_size_local = 0;
+    uint64_t intersection_size_global = 0;
+    uint64_t local_triangles = 0;
+    uint64_t type3_triangles = 0;
+    double preprocessing_time = 0;  // seconds, device
+    double local_phase_time = 0;    // seconds, device
+};
+struct Statistics {
+    uint64_t counted_triangles = 0;
+    LocalStatistics local;
+};
+}  // namespace profiling
+
+// RAII owner of a kt_ctx.
+class Context {
+public:
+    explicit Context(int device = 0, uint64_t hbm_budget_bytes = 0) {
+        kt_ctx* c = nullptr;
+        int rc = kt_create(&c, device, hbm_budget_bytes);
+        if (rc != KT_OK) throw Error(rc, std::string("kt_create: ") + kt_strerror(rc));
+        ctx_.reset(c);
+    }
+    kt_ctx* get() const { return ctx_.get(); }
+    void check(int rc, const char* what) const {
+        if (rc != KT_OK) throw Error(rc, std::string(what) + ": " + kt_last_error(ctx_.get()));
+    }
+
+private:
+    struct Del {
+        void operator()(kt_ctx* c) const { kt_destroy(c); }
+    };
+    std::unique_ptr<kt_ctx, Del> ctx_;
+};
+
+namespace graph {
+// A symmetric CSR held on the host; one rank (rank 0 of 1), as constructed by
+// DistributedGraph(graphio::Graph&&) (distributed_graph.h:797-819).
+class DistributedGraph {
+public:
+    DistributedGraph() : first_out_(1, 0) {}
+    DistributedGraph(std::vector<uint64_t> first_out, std::vector<uint32_t> head)
+        : first_out_(std::move(first_out)), head_(std::move(head)) {
+        if (first_out_.empty() || first_out_.front() != 0 || first_out_.back() != head_.size())
+            throw Error(KT_EINVAL, "DistributedGraph: offsets do not describe the head array");
+    }
+    uint64_t local_node_count() const { return first_out_.size() - 1; }
+    uint64_t local_edge_count() const { return head_.size(); }
+    uint64_t degree(uint64_t v) const { return first_out_[v + 1] - first_out_[v]; }
+    const std::vector<uint64_t>& first_out() const { return first_out_; }
+    const std::vector<uint32_t>& head() const { return head_; }
+
+private:
+    std::vector<uint64_t> first_out_;
+    std::vector<uint32_t> head_;
+};
+}  // namespace graph
+
+using graph::DistributedGraph;
+
+namespace detail {
+inline Context& default_context(const Config& conf) {
+    static Context ctx(conf.device, conf.hbm_budget_bytes);
+    return ctx;
+}
+inline void fill_stats(const Context& ctx, profiling::Statistics& stats, uint64_t counted) {
+    kt_stats s{};
+    kt_timing t{};
+    kt_last_stats(ctx.get(), &s);
+    kt_last_timing(ctx.get(), &t);
+    stats.counted_triangles = counted;
+    stats.local.wedge_checks = s.wedge_checks;
+    stats.local.intersection_size_local = s.intersection_size_local;
+    stats.local.intersection_size_global = s.intersection_size_global;
+    stats.local.local_triangles = s.local_triangles;
+    stats.local.type3_triangles = s.type3_triangles;
+    stats.local.preprocessing_time = t.prep_ms * 1e-3;
+    stats.local.local_phase_time = t.count_ms * 1e-3;
+}
+}  // namespace detail
+
+// cetric::run_shmem: number of triangles of G (preprocessing + local phase on
+// the device).  G is not modified (the reference orients it in place; here the
+// oriented CSR lives in device memory only).
+inline size_t run_shmem(Context& ctx, DistributedGraph& G, profiling::Statistics& stats, const Config& conf) {
+    (void)conf;
+    uint64_t total = 0;
+    ctx.check(kt_count(ctx.get(), (uint32_t)G.local_node_count(), G.first_out().data(), G.head().data(), 0, &total,
+                       nullptr),
+              "kt_count");
+    detail::fill_stats(ctx, stats, total);
+    return total;
+}
+
+inline size_t run_shmem(DistributedGraph& G, profiling::Statistics& stats, const Config& conf) {
+    return run_shmem(detail::default_context(conf), G, stats, conf);
+}
+
+// Total plus per-vertex counts (t[x] = triangles containing x).
+inline uint64_t count_triangles(Context& ctx, const DistributedGraph& G, std::vector<uint64_t>* per_vertex) {
+    uint64_t total = 0;
+    if (per_vertex) per_vertex->assign(G.local_node_count(), 0);
+    ctx.check(kt_count(ctx.get(), (uint32_t)G.local_node_count(), G.first_out().data(), G.head().data(), 0, &total,
+                       per_vertex ? per_vertex->data() : nullptr),
+              "kt_count");
+    return total;
+}
+
+// One rank's share of CeTriC ("cetric-x") or DiTriC ("patric"): the emits of
+// its local phase and of its contracted / distributed phase (run_cetric_new,
+// counters/cetric.h:665-1002; run_patric, :160-316).
+struct RankCounts {
+    uint64_t local_triangles = 0;
+    uint64_t type3_triangles = 0;
+    std::vector<uint64_t> per_vertex;  // owned vertices, then ghosts
+};
+
+inline RankCounts count_rank(Context& ctx, const kt_shard_in& in, bool per_vertex) {
+    RankCounts r;
+    const uint64_t nl = in.range_bounds[in.rank + 1] - in.range_bounds[in.rank];
+    if (per_vertex) r.per_vertex.assign(nl + in.n_ghost, 0);
+    kt_shard_out out{0, 0, per_vertex ? r.per_vertex.data() : nullptr};
+    ctx.check(kt_count_shard(ctx.get(), &in, &out), "kt_count_shard");
+    r.local_triangles = out.local_triangles;
+    r.type3_triangles = out.type3_triangles;
+    return r;
+}
+
+// Many independent graphs in one device pipeline; per-graph totals.
+inline std::vector<uint64_t> count_batch(Context& ctx, const std::vector<DistributedGraph>& graphs) {
+    std::vector<uint64_t> vbase(graphs.size() + 1, 0), off(1, 0);
+    std::vector<uint32_t> heads;
+    for (size_t g = 0; g < graphs.size(); ++g) {
+        const auto& G = graphs[g];
+        vbase[g + 1] = vbase[g] + G.local_node_count();
+        const uint64_t eb = heads.size();
+        for (uint64_t v = 0; v < G.local_node_count(); ++v) off.push_back(eb + G.first_out()[v + 1]);
+        for (uint32_t h : G.head()) heads.push_back((uint32_t)(h + vbase[g]));
+    }
+    std::vector<uint64_t> totals(graphs.size(), 0);
+    ctx.check(kt_batch_count(ctx.get(), (uint32_t)graphs.size(), vbase.data(), off.data(), heads.data(), 0,
+                             totals.data(), nullptr),
+              "kt_batch_count");
+    return totals;
+}
+
+}  // namespace katric

@@ -344,3 +344,7 @@ def run_shmem(g: Graph, ctx: Optional[Context] = None) -> int:
     number of emitted triangles."""
     total, _ = (ctx or default_context()).count(g, per_vertex=False)
     return total
+
+
+# sub-modules register the signatures of their entry points on import
+from . import batch, serve, shard  # noqa: E402,F401

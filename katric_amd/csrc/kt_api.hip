@@ -543,6 +543,26 @@ extern "C" int kt_buffer_download(kt_ctx* c, void* dst, const void* src, uint64_
     return c->hip_check(hipStreamSynchronize(c->stream), "download sync");
 }
 
+extern "C" int kt_host_alloc(uint64_t bytes, void** host_ptr) {
+    if (!host_ptr) return KT_EINVAL;
+    *host_ptr = nullptr;
+    if (hipHostMalloc(host_ptr, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *host_ptr = nullptr;
+        return KT_ENOMEM;
+    }
+    return KT_OK;
+}
+
+extern "C" int kt_host_free(void* host_ptr) {
+    if (!host_ptr) return KT_OK;
+    if (hipHostFree(host_ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return KT_EHIP;
+    }
+    return KT_OK;
+}
+
 extern "C" int kt_synchronize(kt_ctx* c) {
     if (!c) return KT_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -552,6 +572,7 @@ extern "C" int kt_synchronize(kt_ctx* c) {
 }
 
 void kt_ctx::destroy_extensions() {
+    kt_serve_release(this);
     kt_batch_release(this);
     for (auto& kv : buffers) (void)hipFree(kv.first);
     buffers.clear();

@@ -59,29 +59,6 @@ __global__ void k_seg_total(uint32_t G, const uint64_t* __restrict__ vbase, cons
     }
 }
 
-}  // namespace kt
-
-using namespace kt;
-
-namespace {
-
-struct BatchBufs {
-    uint64_t* small;
-    const uint64_t* off;
-    const uint32_t* heads;
-    uint64_t* vbase;
-    uint32_t* deg;
-    uint64_t* ooff;
-    uint32_t* oheads;
-    uint64_t* ws_orient;
-    uint64_t* ws_tc;
-    uint64_t* pv;
-    uint64_t* totals;
-    uint64_t* keys;
-    uint64_t* alt;
-    uint64_t* ws_radix;
-};
-
 uint64_t batch_layout(void* base, uint32_t G, uint64_t N, uint64_t nnz, bool host_in, bool own_pv, bool unsorted,
                       BatchBufs& B) {
     Carver cv{base, 0};
@@ -110,16 +87,16 @@ uint64_t batch_layout(void* base, uint32_t G, uint64_t N, uint64_t nnz, bool hos
     return cv.used;
 }
 
-int bits_for(uint64_t x) {
+static int bits_for(uint64_t x) {
     int b = 0;
     while (b < 64 && (x >> b)) ++b;
     return b;
 }
 
-// Enqueues the device pipeline (no host synchronisation).
-void enqueue_batch(kt_ctx* c, const BatchBufs& B, uint32_t G, uint64_t N, uint64_t nnz, uint64_t* pv,
-                   uint64_t* h_totals, bool sort) {
-    hipStream_t st = c->stream;
+// Enqueues the device pipeline on `st` (no host synchronisation): results land
+// in h_totals (per graph) and h_small ([0] flags, [1] kept edges, [2..3] stats).
+void batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
+                    uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort) {
     uint32_t* err = reinterpret_cast<uint32_t*>(B.small);
     fill_u64(B.small, 64, 0, st);
     orient_count_launch((uint32_t)N, nnz, B.off, B.heads, B.deg, B.ooff, B.ws_orient, err, st);
@@ -136,15 +113,17 @@ void enqueue_batch(kt_ctx* c, const BatchBufs& B, uint32_t G, uint64_t N, uint64
     v.s_off = B.ooff;
     v.sheads = B.oheads;
     TcOut o{pv, B.small + 2};
-    tc_launch(v, o, c->prm, B.ws_tc, c->num_cus, st, nullptr, nullptr);
+    tc_launch(v, o, prm, B.ws_tc, num_cus, st, nullptr, nullptr);
     k_seg_total<<<std::min<uint32_t>(G, 16384), 256, 0, st>>>(G, B.vbase, pv, B.totals, err);
     (void)hipMemcpyAsync(h_totals, B.totals, (uint64_t)G * 8, hipMemcpyDeviceToHost, st);
-    (void)hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st);
-    (void)hipMemcpyAsync(c->host_small + 1, B.ooff + N, 8, hipMemcpyDeviceToHost, st);
-    (void)hipMemcpyAsync(c->host_small + 2, tc_stats_ptr(B.ws_tc), 16, hipMemcpyDeviceToHost, st);
+    (void)hipMemcpyAsync(h_small, B.small, 8, hipMemcpyDeviceToHost, st);
+    (void)hipMemcpyAsync(h_small + 1, B.ooff + N, 8, hipMemcpyDeviceToHost, st);
+    (void)hipMemcpyAsync(h_small + 2, tc_stats_ptr(B.ws_tc), 16, hipMemcpyDeviceToHost, st);
 }
 
-}  // namespace
+}  // namespace kt
+
+using namespace kt;
 
 void kt_batch_release(kt_ctx* c) {
     if (!c->batch) return;
@@ -232,7 +211,7 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
                 hipGraph_t graph = nullptr;
                 rc = c->hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "begin capture");
                 if (rc) return rc;
-                enqueue_batch(c, B, G, N, nnz, pv, S->h_totals, false);
+                batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small, false);
                 rc = c->hip_check(hipStreamEndCapture(st, &graph), "end capture");
                 if (rc) return rc;
                 rc = c->hip_check(hipGraphInstantiate(&S->exec, graph, nullptr, nullptr, 0), "instantiate");
@@ -257,7 +236,7 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
             S->replays++;
         } else {
             hipEventRecord(c->ev[EV_PREP], st);
-            enqueue_batch(c, B, G, N, nnz, pv, S->h_totals, true);
+            batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small, true);
         }
         hipEventRecord(c->ev[EV_COUNT], st);
         if (want_pv && !pv_dev) {

@@ -62,6 +62,7 @@ struct kt_ctx {
 };
 
 void kt_batch_release(kt_ctx* c);
+void kt_serve_release(kt_ctx* c);
 
 namespace kt {
 uint64_t* tc_stats_ptr(uint64_t* ws);

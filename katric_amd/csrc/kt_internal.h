@@ -70,6 +70,29 @@ struct TcOut {
     uint64_t* totals;           // per graph (>= 1 entry)
 };
 
+// Batch pipeline over a disjoint-union CSR (kt_batch.hip), shared with the
+// serving loop.
+struct BatchBufs {
+    uint64_t* small;
+    const uint64_t* off;
+    const uint32_t* heads;
+    uint64_t* vbase;
+    uint32_t* deg;
+    uint64_t* ooff;
+    uint32_t* oheads;
+    uint64_t* ws_orient;
+    uint64_t* ws_tc;
+    uint64_t* pv;
+    uint64_t* totals;
+    uint64_t* keys;
+    uint64_t* alt;
+    uint64_t* ws_radix;
+};
+uint64_t batch_layout(void* base, uint32_t G, uint64_t N, uint64_t nnz, bool host_in, bool own_pv, bool unsorted,
+                      BatchBufs& B);
+void batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
+                    uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort);
+
 uint64_t tc_ws_words(uint32_t X);
 void tc_launch(const TcView& view, const TcOut& out, const TcParams& prm, uint64_t* ws, int num_cus,
                hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end);

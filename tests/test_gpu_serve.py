@@ -1,0 +1,107 @@
+"""Serving loop: every result equals the one-graph count, alone and in the mix;
+admission by budget, isolation of malformed requests, cancellation."""
+import time
+
+import numpy as np
+import pytest
+
+import katric_amd as K
+from katric_amd import serve as KS
+
+pytestmark = pytest.mark.gpu
+
+
+def mix(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        if rng.random() < 0.5:
+            out.append(K.gen_rmat(int(rng.integers(6, 14)), int(rng.integers(4, 17)), seed=int(rng.integers(1 << 30))))
+        else:
+            n = int(rng.integers(50, 20000))
+            out.append(K.gen_gnm(n, int(n * rng.uniform(2, 16)), seed=int(rng.integers(1 << 30))))
+    return out
+
+
+def test_serve_results_equal_alone(ctx):
+    graphs = mix(40, 1)
+    alone = [ctx.count(g) for g in graphs]
+    with K.Context(0) as sctx:
+        srv = KS.Server(sctx)
+        pvs = [np.zeros(g.n, np.uint64) for g in graphs]
+        tickets = [srv.submit(g, pv) for g, pv in zip(graphs, pvs)]
+        results = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+        srv.stop()
+    for (t, pv), r, p in zip(alone, results, pvs):
+        assert r.status == 0 and r.total == t
+        np.testing.assert_array_equal(p, pv)
+    assert st["graphs"] == 40
+    assert max(r.batch_graphs for r in results) > 1, "requests were batched"
+
+
+def test_serve_arrivals_during_batches(ctx):
+    graphs = mix(60, 2)
+    alone = [ctx.count(g, per_vertex=False)[0] for g in graphs]
+    with K.Context(0) as sctx:
+        srv = KS.Server(sctx)
+        tickets = []
+        for i, g in enumerate(graphs):
+            tickets.append(srv.submit(g))
+            if i % 7 == 0:
+                time.sleep(0.002)
+        res = [srv.wait(t) for t in tickets]
+        srv.stop()
+    assert [r.total for r in res] == alone
+    assert len({r.batch_id for r in res}) > 1
+
+
+def test_serve_isolates_malformed(ctx):
+    graphs = mix(10, 3)
+    bad = K.Graph(3, np.array([0, 1, 1, 2], np.uint64), np.array([1, 1], np.uint32))  # asymmetric
+    out_of_range = K.Graph(2, np.array([0, 1, 2], np.uint64), np.array([1, 7], np.uint32))
+    alone = [ctx.count(g, per_vertex=False)[0] for g in graphs]
+    with K.Context(0) as sctx:
+        srv = KS.Server(sctx)
+        tk = [srv.submit(g) for g in graphs[:5]] + [srv.submit(bad), srv.submit(out_of_range)] + \
+             [srv.submit(g) for g in graphs[5:]]
+        res = [srv.wait(t) for t in tk]
+        srv.stop()
+    good = res[:5] + res[7:]
+    assert [r.total for r in good] == alone and all(r.status == 0 for r in good)
+    assert res[5].status == K.KT_EINVAL and res[6].status == K.KT_EINVAL
+
+
+def test_serve_budget_admission(ctx):
+    graphs = mix(20, 4)
+    big = K.gen_rmat(16, 16, seed=1)
+    alone = [ctx.count(g, per_vertex=False)[0] for g in graphs]
+    with K.Context(0) as sctx:
+        srv = KS.Server(sctx, hbm_budget_bytes=24 << 20)  # 12 MiB per batch slot; RMAT-16 needs ~17 MiB
+        tk = [srv.submit(g) for g in graphs]
+        tbig = srv.submit(big)
+        res = [srv.wait(t) for t in tk]
+        rbig = srv.wait(tbig)
+        srv.stop()
+    assert [r.total for r in res] == alone
+    assert rbig.status == K.KT_ENOMEM
+    assert max(r.batch_graphs for r in res) < 20  # the budget split the stream into several batches
+
+
+def test_serve_cancel_and_unsorted(ctx):
+    g = K.gen_rmat(11, 16, seed=9)
+    rng = np.random.default_rng(0)
+    heads = g.heads.copy()
+    for v in range(g.n):
+        a, b = int(g.offsets[v]), int(g.offsets[v + 1])
+        heads[a:b] = rng.permutation(heads[a:b])
+    ug = K.Graph(g.n, g.offsets, heads)
+    want = ctx.count(g, per_vertex=False)[0]
+    with K.Context(0) as sctx:
+        srv = KS.Server(sctx)
+        tk = [srv.submit(ug) for _ in range(30)]
+        srv.cancel(tk[-1])
+        res = [srv.wait(t) for t in tk]
+        srv.stop()
+    assert all(r.total == want and r.status == 0 for r in res[:-1])
+    assert res[-1].status in (0, K.KT_ECANCELED)
