@@ -39,11 +39,13 @@ constexpr int NBUCKET = 13;  // bucket b holds tails with |S| in (2^(b-1), 2^b];
 constexpr int BUCKET_HUGE = 12;
 constexpr uint32_t WAVE_GRAB = 8;  // wave-phase tails taken per queue access
 
-// work-queue header (u32 words at the start of the workspace):
-//   [0, 13)   tails per bucket
-//   [16, 29)  pull cursor per bucket
-//   [32, 45)  fill cursor per bucket (binning pass 2)
-enum : int { Q_CNT = 0, Q_PULL = 16, Q_FILL = 32, Q_WORDS = 64 };
+// work-queue header (u32 words at the start of the workspace); every counter
+// that waves update concurrently sits on its own 128-byte line so that the
+// queues of different buckets land on different L2 channels:
+//   [0, 13)               tails per bucket (written by binning, then read-only)
+//   Q_PULL + 32 * b       pull cursor of bucket b
+//   Q_FILL + 32 * b       fill cursor of bucket b (binning pass 2)
+enum : int { Q_CNT = 0, Q_PULL = 32, Q_FILL = 32 * 14, Q_STRIDE = 32, Q_WORDS = 32 * 28 };
 enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_WORDS = 4 };
 
 template <bool PLAIN>
@@ -180,7 +182,7 @@ __global__ void __launch_bounds__(256) k_tc_bin_fill(View<PLAIN> V, TcParams prm
     __syncthreads();
     if (threadIdx.x < NBUCKET) {
         const uint32_t c = cnt[threadIdx.x];
-        pos[threadIdx.x] = c ? bucket_base(q, threadIdx.x) + atomicAdd(&q[Q_FILL + threadIdx.x], c) : 0;
+        pos[threadIdx.x] = c ? bucket_base(q, threadIdx.x) + atomicAdd(&q[Q_FILL + Q_STRIDE * threadIdx.x], c) : 0;
     }
     __syncthreads();
     for (uint32_t x = x0 + threadIdx.x; x < x1; x += blockDim.x) {
@@ -514,8 +516,12 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
         const uint32_t cnt = q[Q_CNT + b];
         if (cnt == 0) continue;
         const uint32_t base = bucket_base(q, b);
+        uint32_t* pull = &q[Q_PULL + Q_STRIDE * b];
         while (true) {
-            if (threadIdx.x == 0) L.bcast = atomicAdd(&q[Q_PULL + b], 1u);
+            if (threadIdx.x == 0)
+                L.bcast = (__hip_atomic_load(pull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt)
+                              ? cnt
+                              : atomicAdd(pull, 1u);
             __syncthreads();
             const uint32_t i = L.bcast;
             __syncthreads();
@@ -532,23 +538,44 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             const uint32_t cnt = q[Q_CNT + b];
             if (cnt == 0) continue;
             const uint32_t base = bucket_base(q, b);
+            uint32_t* pull = &q[Q_PULL + Q_STRIDE * b];
+            // more tails per grab for the small buckets (less queue traffic)
+            const uint32_t grab = b <= 3 ? 4 * WAVE_GRAB : (b <= 5 ? 2 * WAVE_GRAB : WAVE_GRAB);
             while (true) {
                 uint32_t i = 0;
-                if (lane == 0) i = atomicAdd(&q[Q_PULL + b], WAVE_GRAB);
+                if (lane == 0)
+                    i = (__hip_atomic_load(pull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt)
+                            ? cnt
+                            : atomicAdd(pull, grab);
                 i = uniform(i);
                 if (i >= cnt) break;
-                const uint32_t e = min(cnt, i + WAVE_GRAB);
+                const uint32_t e = min(cnt, i + grab);
                 for (; i < e; ++i)
                     team_process<1, true, PLAIN>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, KT_WAVE, out,
                                                  my_total, st_wedge, st_isz);
             }
         }
     }
-    // flush per-wave accumulators (wave-uniform values)
+    // flush the wave-uniform accumulators: one atomic per workgroup and counter
+    __syncthreads();
     if (lane == 0) {
-        if (my_total) atomic_add_u64(&out.totals[0], my_total);
-        if (st_wedge) atomic_add_u64(&stats[ST_WEDGE], st_wedge);
-        if (st_isz) atomic_add_u64(&stats[ST_ISZ], st_isz);
+        L.red[wave] = my_total;
+        L.keys[wave] = (uint32_t)st_wedge;
+        L.keys[TC_WAVES + wave] = (uint32_t)(st_wedge >> 32);
+        L.cnt[wave] = (uint32_t)st_isz;
+        L.cnt[TC_WAVES + wave] = (uint32_t)(st_isz >> 32);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0, w = 0, z = 0;
+        for (int i = 0; i < TC_WAVES; ++i) {
+            t += L.red[i];
+            w += ((uint64_t)L.keys[TC_WAVES + i] << 32) | L.keys[i];
+            z += ((uint64_t)L.cnt[TC_WAVES + i] << 32) | L.cnt[i];
+        }
+        if (t) atomic_add_u64(&out.totals[0], t);
+        if (w) atomic_add_u64(&stats[ST_WEDGE], w);
+        if (z) atomic_add_u64(&stats[ST_ISZ], z);
     }
 }
 
@@ -563,8 +590,12 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
     uint64_t my_total = 0, st_wedge = 0, st_isz = 0;
     const uint32_t cnt = q[Q_CNT + BUCKET_HUGE];
     const uint32_t base = bucket_base(q, BUCKET_HUGE);
+    uint32_t* pull = &q[Q_PULL + Q_STRIDE * BUCKET_HUGE];
     while (true) {
-        if (threadIdx.x == 0) L.bcast = atomicAdd(&q[Q_PULL + BUCKET_HUGE], 1u);
+        if (threadIdx.x == 0)
+            L.bcast = (__hip_atomic_load(pull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt)
+                          ? cnt
+                          : atomicAdd(pull, 1u);
         __syncthreads();
         const uint32_t i = L.bcast;
         __syncthreads();
