@@ -42,6 +42,7 @@ inline uint32_t below32(uint32_t r, uint64_t n) { return (uint32_t)(((uint64_t)r
 
 int num_threads() {
     const char* e = getenv("KATRIC_THREADS");
+    if (!e) e = getenv("OMP_NUM_THREADS");  // the host's CPU share, where a scheduler sets it
     int t = e ? atoi(e) : (int)std::thread::hardware_concurrency();
     if (t < 1) t = 1;
     if (t > 64) t = 64;

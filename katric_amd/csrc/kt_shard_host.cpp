@@ -37,6 +37,7 @@ inline int owner_of(uint64_t x, const uint64_t* bounds, int P) {
 
 int nthreads() {
     const char* e = getenv("KATRIC_THREADS");
+    if (!e) e = getenv("OMP_NUM_THREADS");  // the host's CPU share, where a scheduler sets it
     int t = e ? atoi(e) : (int)std::thread::hardware_concurrency();
     return std::max(1, std::min(t, 64));
 }

@@ -127,3 +127,42 @@ def count_shard(ctx, s: Shard, per_vertex: bool = True):
     so = KtShardOut(0, 0, pv.ctypes.data if pv is not None and pv.size else None)
     ctx.check(lib().kt_count_shard(ctx.handle, C.byref(si), C.byref(so)), "kt_count_shard")
     return int(so.local_triangles), int(so.type3_triangles), pv
+
+
+@dataclass
+class DeviceShard:
+    """A shard's input arrays resident in device memory (range bounds stay on the host)."""
+    shard: Shard
+    bufs: dict
+
+
+def upload_shard(ctx, s: Shard) -> DeviceShard:
+    bufs = {}
+    for name in ("offsets", "heads", "ghost_ids", "ghost_degree", "msg_src", "msg_offsets", "msg_heads"):
+        a = getattr(s, name)
+        b = ctx.alloc(max(a.nbytes, 8))
+        if a.size:
+            b.upload(a)
+        bufs[name] = b
+    return DeviceShard(s, bufs)
+
+
+def count_shard_device(ctx, d: DeviceShard, per_vertex_out=None):
+    """kt_count_shard on device-resident inputs (KT_DEVICE_INPUT); per-vertex
+    counts go to the device buffer `per_vertex_out` when given.  Returns
+    (local_triangles, type3_triangles)."""
+    from . import KT_DEVICE_INPUT, KT_DEVICE_OUTPUT
+
+    s, b = d.shard, d.bufs
+    flags = KT_DEVICE_INPUT | (KT_DEVICE_OUTPUT if per_vertex_out is not None else 0)
+    si = KtShardIn(
+        algorithm=s.algorithm, rank=s.rank, num_ranks=s.num_ranks, flags=flags,
+        range_bounds=s.bounds.ctypes.data, offsets=b["offsets"].ptr, heads=b["heads"].ptr,
+        n_ghost=len(s.ghost_ids), ghost_ids=b["ghost_ids"].ptr, ghost_degree=b["ghost_degree"].ptr,
+        n_msg=len(s.msg_src), msg_src=b["msg_src"].ptr, msg_offsets=b["msg_offsets"].ptr,
+        msg_heads=b["msg_heads"].ptr,
+    )
+    pv = per_vertex_out.ptr if per_vertex_out is not None else None
+    so = KtShardOut(0, 0, pv)
+    ctx.check(lib().kt_count_shard(ctx.handle, C.byref(si), C.byref(so)), "kt_count_shard")
+    return int(so.local_triangles), int(so.type3_triangles)
