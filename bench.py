@@ -38,6 +38,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 
+# HBM bytes per k_tc_main launch, measured with two rocprofv3 --pmc runs of
+# this workload (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch).  A plain
+# 1 GiB device copy reads FETCH_SIZE = 524,308 KiB and WRITE_SIZE =
+# 1,048,576 KiB on MI355X, so FETCH_SIZE is scaled by 2.000 and WRITE_SIZE by
+# 1.000: 58,503,750 KiB x 2.000 + 4,430,501 KiB = 124.35 GB.
+MEASURED_TRAFFIC = {("plain", 22, 16, 1): 124.35e9}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -60,10 +67,20 @@ def parse():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--params", default="",
+                    help="intersection knobs, e.g. strategy=1,wave_max_deg=64 (kt_params fields)")
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="omit the CPU reference timing (profiling runs)")
     ap.add_argument("--no-check", action="store_true", help="omit the total cross-check against the CPU run")
     return ap.parse_args()
+
+
+def apply_params(ctx, args):
+    """Applies --params to the context; returns the knobs in effect."""
+    if args.params:
+        kv = dict(item.split("=") for item in args.params.split(","))
+        ctx.set_params(**{k.strip(): int(v) for k, v in kv.items()})
+    return ctx.params()
 
 
 def log(*a):
@@ -104,6 +121,7 @@ def run_shard(args):
     log(f"[bench] shard {args.rank}/{args.ranks}: owned {s.n_local}, local heads {len(s.heads)}, "
         f"ghosts {len(s.ghost_ids)}, messages {len(s.msg_src)} ({mtot} ids), make {time.time() - t0:.1f}s")
     ctx = K.Context(0)
+    knobs = apply_params(ctx, args)
     ds = S.upload_shard(ctx, s)
     X = s.n_local + len(s.ghost_ids)
     d_pv = ctx.alloc(max(X, 1) * 8)
@@ -144,6 +162,7 @@ def run_shard(args):
         "dtype": "u32 ids / u64 counts",
         "data": "synthetic (seeded RMAT generator, permuted ids); shard inputs built on the host from the full graph",
         "config": {
+            "params": knobs,
             "workload": f"RMAT-{args.scale} ef{args.edge_factor} seed={args.seed}, {args.ranks} contiguous ranges, "
                         f"rank {args.rank}: local CSR + ghost degrees + N+cut messages resident in HBM -> "
                         f"local / type-3 totals + per-vertex uint64 in HBM",
@@ -243,6 +262,7 @@ def run_serve(args):
     R = args.requests
     which = rng.integers(0, len(pool), size=R)
     ctx = K.Context(0)
+    knobs = apply_params(ctx, args)
     # reference totals, one kt_count per pool graph (also warms the plain path)
     want = np.array([ctx.count(g, per_vertex=False)[0] for g in pool], np.uint64)
 
@@ -316,6 +336,7 @@ def run_serve(args):
         "data": f"synthetic: requests drawn from a seeded pool of {len(pool)} G(n,m)/RMAT graphs "
                 f"(n log-uniform [2^10, 2^20], avg degree U[4,32]) in page-locked host memory",
         "config": {
+            "params": knobs,
             "workload": f"{R} requests, Poisson arrivals at {rate:.0f}/s, CSR uploaded per request (PCIe inclusive), "
                         f"HBM-budget admission, continuous batching",
             "arrival_rate_per_s": rate, "requests": R, "edges": edges, "edges_per_s": edges / wall,
@@ -344,6 +365,7 @@ def run_batch(args):
     G, N, E = b.num_graphs, b.n_total, b.nnz // 2
     log(f"[bench] batch: {G} graphs, {N} vertices, {E} edges, gen {time.time() - t0:.1f}s")
     ctx = K.Context(0)
+    knobs = apply_params(ctx, args)
     db = KB.upload_batch(ctx, b)
     pv = ctx.alloc(max(N, 1) * 8)
     # batched: the first call captures the pipeline, the rest replay it
@@ -398,6 +420,7 @@ def run_batch(args):
         "dtype": "u32 ids / u64 counts",
         "data": "synthetic (seeded mixed batch generator)",
         "config": {
+            "params": knobs,
             "workload": f"{G} graphs, n log-uniform [64, 65536], avg degree U[4,64], G(n,m)/RMAT 50/50, "
                         f"seed={args.seed}; disjoint-union CSR resident in HBM -> per-graph uint64 totals",
             "graphs": G, "vertices": N, "edges": E, "triangles": int(tot.sum()),
@@ -422,6 +445,7 @@ def run_plain(args):
     checksum = g.checksum()
 
     ctx = K.Context(0)
+    knobs = apply_params(ctx, args)
     dg = ctx.upload(g)          # the symmetric CSR resident in HBM
     d_pv = ctx.alloc(n * 8)     # per-vertex uint64 counts, in HBM
     ctx.synchronize()
@@ -471,6 +495,7 @@ def run_plain(args):
         "dtype": "u32 ids / u64 counts",
         "data": "synthetic (seeded RMAT generator, permuted ids)",
         "config": {
+            "params": knobs,
             "workload": f"RMAT-{args.scale} ef{args.edge_factor} a=0.57 b=0.19 c=0.19 seed={args.seed}, "
                         f"symmetric CSR resident in HBM -> total + per-vertex uint64 in HBM",
             "n": n, "m": m, "triangles": total, "W": W, "csr_checksum": f"{checksum:016x}",
@@ -485,7 +510,7 @@ def run_plain(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get(("plain", args.scale, args.edge_factor, args.seed)),
             "kernel_ms": k_ms,
         },
         "cpu_baseline": None,

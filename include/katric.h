@@ -73,13 +73,16 @@ typedef struct {
     float d2h_ms;       /* device -> host copies of the outputs */
 } kt_timing;
 
-/* Intersection strategy knobs (bin thresholds); see kt_tune_* in the
- * package for the autotuner that chooses them. */
+/* Intersection strategy knobs (bin thresholds and list walk); the package's
+ * autotuner (katric_amd.tune) chooses them per graph class.  Counts never
+ * depend on them. */
+#define KT_STRATEGY_STREAMED 0u /* one T list at a time, aligned 16-byte blocks (long lists) */
+#define KT_STRATEGY_PACKED 1u   /* a chunk's T lists concatenated into 256-id windows (short lists) */
 typedef struct {
     uint32_t wave_max_deg;  /* |N+(v)| <= this: one wave per tail, LDS hash (<= 256) */
     uint32_t group_max_deg; /* |N+(v)| <= this: one workgroup per tail, LDS hash (<= 2048) */
-    uint32_t grid_per_cu;   /* persistent 512-thread workgroups per compute unit */
-    uint32_t reserved;
+    uint32_t grid_per_cu;   /* persistent 512-thread workgroups per compute unit (1..4 fill the CU) */
+    uint32_t strategy;      /* KT_STRATEGY_* */
 } kt_params;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes bounds the

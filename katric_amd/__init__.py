@@ -41,6 +41,8 @@ __all__ = [
     "KT_DEVICE_OUTPUT",
 ]
 
+STRATEGY_STREAMED = 0  # KT_STRATEGY_STREAMED
+STRATEGY_PACKED = 1  # KT_STRATEGY_PACKED
 KT_DEVICE_INPUT = 0x1
 KT_DEVICE_OUTPUT = 0x2
 
@@ -250,13 +252,17 @@ class Context:
     def params(self) -> dict:
         p = KtParams()
         self.check(lib().kt_get_params(self._h, C.byref(p)), "kt_get_params")
-        return {"wave_max_deg": p.wave_max_deg, "group_max_deg": p.group_max_deg, "grid_per_cu": p.grid_per_cu}
+        return {f: getattr(p, f) for f, _ in KtParams._fields_}
 
-    def set_params(self, wave_max_deg=None, group_max_deg=None, grid_per_cu=None):
+    def set_params(self, wave_max_deg=None, group_max_deg=None, grid_per_cu=None, strategy=None, **extra):
+        """Intersection knobs (see kt_params); unknown keys are rejected."""
+        if extra:
+            raise TypeError(f"unknown parameters: {sorted(extra)}")
         cur = self.params()
-        p = KtParams(wave_max_deg if wave_max_deg is not None else cur["wave_max_deg"],
-                     group_max_deg if group_max_deg is not None else cur["group_max_deg"],
-                     grid_per_cu if grid_per_cu is not None else cur["grid_per_cu"], 0)
+        new = dict(wave_max_deg=wave_max_deg, group_max_deg=group_max_deg, grid_per_cu=grid_per_cu,
+                   strategy=strategy)
+        cur.update({k: int(v) for k, v in new.items() if v is not None})
+        p = KtParams(**cur)
         self.check(lib().kt_set_params(self._h, C.byref(p)), "kt_set_params")
 
     def stats(self) -> dict:

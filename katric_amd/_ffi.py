@@ -47,7 +47,7 @@ class KtParams(C.Structure):
         ("wave_max_deg", u32),
         ("group_max_deg", u32),
         ("grid_per_cu", u32),
-        ("reserved", u32),
+        ("strategy", u32),
     ]
 
 

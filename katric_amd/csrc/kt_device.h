@@ -122,6 +122,21 @@ __device__ __forceinline__ void tile_rows(const uint64_t* __restrict__ off, uint
     __syncthreads();
 }
 
+// Inclusive max-scan across the wave with DPP row shifts and row broadcasts
+// (no LDS traffic): within each 16-lane row by shifts of 1, 2, 4, 8 (lanes
+// shifted in from outside the row read 0, the identity of max over unsigned
+// values), then lane 15 of rows 0 and 2 into rows 1 and 3, then lane 31 into
+// rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_incl_max_dpp(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));  // row_shr:1
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));  // row_shr:2
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));  // row_shr:4
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));  // row_shr:8
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false)); // row_bcast:15
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false)); // row_bcast:31
+    return v;
+}
+
 // Multiplicative (Fibonacci) hash into a 2^logS table.
 __device__ __forceinline__ uint32_t hash_slot(uint32_t w, uint32_t logS) {
     return (w * 2654435761u) >> (32 - logS);

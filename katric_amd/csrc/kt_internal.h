@@ -38,10 +38,16 @@ void fill_u32(uint32_t* p, uint64_t count, uint32_t value, hipStream_t st);
 // Intersection tuning knobs (bin thresholds, table sizes); see kt_tc.hip.
 struct TcParams {
     uint32_t wave_max_deg;    // d+(v) <= this: one wave per tail vertex (LDS hash)
-    uint32_t group_max_deg;   // d+(v) <= this: one workgroup (4 waves) per tail vertex (LDS hash)
+    uint32_t group_max_deg;   // d+(v) <= this: one workgroup (8 waves) per tail vertex (LDS hash)
     uint32_t team_chunk;      // neighbours per work grab inside a workgroup team (1..64)
     uint32_t grid_per_cu;     // persistent workgroups per CU
+    uint32_t strategy;        // TC_STREAMED or TC_PACKED (hash-probe tails)
 };
+
+// How a team walks the T lists of a chunk of neighbours: one list at a time
+// in aligned 16-byte blocks (TC_STREAMED), or all lists of the chunk
+// concatenated into 256-id windows (TC_PACKED, for short lists).
+enum : uint32_t { TC_STREAMED = 0, TC_PACKED = 1 };
 
 // Generic "tail list" view for the intersection kernel:
 //   tails x in [0, X): hash set S(x) = sheads[s_off[x] .. s_off[x+1]),
@@ -68,6 +74,7 @@ struct TcView {
 struct TcOut {
     uint64_t* per_vertex;       // nullable
     uint64_t* totals;           // per graph (>= 1 entry)
+    uint64_t* err = nullptr;    // set by tc_launch: the workspace's error word
 };
 
 // Batch pipeline over a disjoint-union CSR (kt_batch.hip), shared with the

@@ -46,7 +46,10 @@ constexpr uint32_t WAVE_GRAB = 8;  // wave-phase tails taken per queue access
 //   Q_PULL + 32 * b       pull cursor of bucket b
 //   Q_FILL + 32 * b       fill cursor of bucket b (binning pass 2)
 enum : int { Q_CNT = 0, Q_PULL = 32, Q_FILL = 32 * 14, Q_STRIDE = 32, Q_WORDS = 32 * 28 };
-enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_WORDS = 4 };
+// stats words after the queue header: work counters, then an error word
+// (non-zero: a chunk's packed stream exceeded 32-bit positions; the counts of
+// that launch are incomplete)
+enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_ERR = 2, ST_WORDS = 4 };
 
 template <bool PLAIN>
 struct View {
@@ -195,9 +198,17 @@ __global__ void __launch_bounds__(256) k_tc_bin_fill(View<PLAIN> V, TcParams prm
 // ---------------------------------------------------------------------------
 // the team worker
 // ---------------------------------------------------------------------------
+// Per-wave scratch of the packed scan: start marks of one 256-id window and
+// the hit counters of the (up to 64) neighbours of the current chunk.
+struct PackLds {
+    uint32_t marks[KT_WAVE];
+    uint32_t ucnt[KT_WAVE];
+};
+
 struct Lds {
     uint32_t keys[GROUP_SLOTS];
     uint32_t cnt[GROUP_SLOTS];
+    PackLds pk[TC_WAVES];
     uint64_t red[TC_WAVES];
     uint32_t next_u;
     uint32_t bcast;
@@ -234,8 +245,8 @@ __device__ __forceinline__ uint32_t match4(const uint4& k, uint32_t w) {
     return (k.x == w ? 1u : 0u) | (k.y == w ? 2u : 0u) | (k.z == w ? 4u : 0u) | (k.w == w ? 8u : 0u);
 }
 
-// Probes up to 4 ids of this lane (valid bits in vmask); returns the number of
-// hits and bumps each hit slot's w-counter.
+// Probes up to 4 ids of this lane (valid bits in vmask); returns the mask of
+// the ids found and bumps each hit slot's w-counter.
 __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, uint32_t logB, uint32_t bmask,
                                            const uint32_t (&w)[4], uint32_t vmask) {
     const uint4* kb = reinterpret_cast<const uint4*>(keys);
@@ -253,7 +264,7 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
         const bool valid = (vmask >> e) & 1u;
         if (valid && m) {
             atomicAdd(&cnt[4 * bk[e] + __builtin_ctz(m)], 1u);
-            ++hits;
+            hits |= 1u << e;
         }
         if (valid && !m && kk[e].w != KT_EMPTY) cont |= 1u << e;
     }
@@ -275,7 +286,7 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
                 const uint32_t m = match4(k, wb);
                 if (m) {
                     atomicAdd(&cnt[4 * b + __builtin_ctz(m)], 1u);
-                    ++hits;
+                    hits |= 1u << e;
                     break;
                 }
                 if (k.w == KT_EMPTY) break;
@@ -320,6 +331,7 @@ struct ProbeCtx {
     const uint32_t* s;  // sorted S(x) (binary-search probes)
     uint64_t ns;
     uint64_t* pv;
+    uint64_t* err;
 };
 
 // Streams the T lists of up to 64 neighbours R[k0, k0+cnt) through the probe.
@@ -386,7 +398,7 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
 #pragma unroll
         for (int e = 0; e < 4; ++e) vmask |= (rel0 + e < len) ? (1u << e) : 0u;
         if (HASH) {
-            hits += probe4(P.keys, P.cnt, P.logB, P.bmask, w, vmask);
+            hits += __builtin_popcount(probe4(P.keys, P.cnt, P.logB, P.bmask, w, vmask));
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e)
@@ -410,8 +422,108 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
     return total;
 }
 
+// Packed variant of scan_chunk for the hash probe: the T lists of the chunk's
+// neighbours are concatenated into one stream of `total` ids, and the wave
+// walks the stream in windows of 256 ids -- id e * 64 + lane of a window is
+// lane `lane`'s e-th id, so each of the four loads is one contiguous run of
+// 64 ids per list piece.  Short lists therefore share a window instead of
+// taking one each.  The list holding stream position p is the last list
+// starting at or before p: lists mark their start in the window (one byte in
+// LDS), and a max-scan over the window positions carries the latest mark
+// forward.  Hits are attributed to their neighbour u through per-wave LDS
+// counters, flushed with one global atomic per neighbour.
+template <bool PLAIN>
+__device__ __forceinline__ uint64_t scan_chunk_packed(const View<PLAIN>& V, uint64_t k0, uint32_t cnt,
+                                                      const ProbeCtx& P, uint64_t ns, uint64_t& st_wedge,
+                                                      uint64_t& st_isz, PackLds& pk) {
+    const uint32_t lane = lane_id();
+    uint32_t uj = 0;
+    uint64_t tbj = 0, tej = 0;
+    if (lane < cnt) {
+        uj = V.r_head(k0 + lane);
+        V.t_range(uj, tbj, tej);
+    }
+    const uint64_t len64 = tej - tbj;
+    const uint64_t incl64 = wave_incl_sum(len64);
+    const uint64_t total64 = uniform64(__builtin_amdgcn_readlane((uint32_t)incl64, KT_WAVE - 1) |
+                                       ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(incl64 >> 32), KT_WAVE - 1)
+                                        << 32));
+    if (total64 >= (1ull << 31)) {
+        // positions below are 32-bit; under the degree order |N+(u)| <=
+        // sqrt(2m), so only inconsistent shard inputs (ghost degrees) get here
+        if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(P.err), 1ull);
+        return 0;
+    }
+    st_wedge += total64;
+    st_isz += total64 + (uint64_t)cnt * ns;
+    if (total64 == 0) return 0;
+    const uint32_t total = (uint32_t)total64;
+    const uint32_t len = (uint32_t)len64;
+    const uint32_t incl = (uint32_t)incl64;
+    const uint32_t start = incl - len;
+    const uint64_t base = tbj - start;  // stream position p of list j sits at T[base_j + p]
+    const uint32_t base_lo = (uint32_t)base, base_hi = (uint32_t)(base >> 32);
+    const uint32_t* __restrict__ T = V.t_heads();
+    uint8_t* marks8 = reinterpret_cast<uint8_t*>(pk.marks);
+    pk.ucnt[lane] = 0;
+    uint32_t carry = 0;  // 1 + the list active at the start of the next window
+
+    // resolve the lists of window w0 and load its ids
+    auto window = [&](uint32_t w0, uint32_t (&w)[4], uint32_t (&jj)[4], uint32_t& vmask) {
+        pk.marks[lane] = 0;
+        wave_sync();
+        if (len != 0 && start >= w0 && start - w0 < 4 * KT_WAVE) {
+            const uint32_t q = start - w0;  // window position q lives at byte (q % 64) * 4 + q / 64
+            marks8[(q & (KT_WAVE - 1)) * 4 + (q >> 6)] = (uint8_t)(lane + 1);
+        }
+        wave_sync();
+        const uint32_t m4 = pk.marks[lane];
+        vmask = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            uint32_t v = wave_incl_max_dpp((m4 >> (8 * e)) & 0xffu);
+            v = max(v, carry);
+            carry = uniform(__builtin_amdgcn_readlane(v, KT_WAVE - 1));
+            const uint32_t pos = w0 + e * KT_WAVE + lane;
+            const int src = (int)((v ? v - 1 : 0u) * 4);
+            const uint32_t blo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)base_lo);
+            const uint32_t bhi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)base_hi);
+            const uint32_t iend = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)incl);
+            const uint32_t ibeg = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)start);
+            // v > 0 and ibeg <= pos < iend: list j holds pos, so base_j + pos
+            // lies inside T(u_j)
+            const bool ok = v != 0 && pos < total && pos >= ibeg && pos < iend;
+            jj[e] = v - 1;
+            w[e] = KT_EMPTY;
+            if (ok) w[e] = T[(((uint64_t)bhi << 32) | blo) + pos];
+            vmask |= (ok ? 1u : 0u) << e;
+        }
+    };
+
+    uint32_t w[4], jj[4], vm;
+    window(0, w, jj, vm);
+    for (uint32_t w0 = 0; w0 < total; w0 += 4 * KT_WAVE) {
+        uint32_t wn[4] = {KT_EMPTY, KT_EMPTY, KT_EMPTY, KT_EMPTY}, jn[4] = {0, 0, 0, 0}, vn = 0;
+        if (w0 + 4 * KT_WAVE < total) window(w0 + 4 * KT_WAVE, wn, jn, vn);  // prefetch the next window
+        const uint32_t hm = probe4(P.keys, P.cnt, P.logB, P.bmask, w, vm);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if ((hm >> e) & 1u) atomicAdd(&pk.ucnt[jj[e]], 1u);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            w[e] = wn[e];
+            jj[e] = jn[e];
+        }
+        vm = vn;
+    }
+    wave_sync();
+    const uint32_t h = lane < cnt ? pk.ucnt[lane] : 0u;
+    if (h && P.pv) atomic_add_u64(&P.pv[uj], h);
+    return uniform(wave_sum(h));
+}
+
 // Process tail x with a team of NW waves (NW == 1: this wave alone).
-template <int NW, bool HASH, bool PLAIN>
+template <int NW, bool HASH, bool PLAIN, bool PACK>
 __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, Lds& L, uint32_t* keys,
                                              uint32_t* cnt, uint32_t cap_slots, uint32_t chunk, const TcOut& out,
                                              uint64_t& my_total, uint64_t& st_wedge, uint64_t& st_isz) {
@@ -437,6 +549,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     P.s = V.v.sheads + sb;
     P.ns = ns;
     P.pv = out.per_vertex;
+    P.err = out.err;
     uint32_t slots = 0;
     if (HASH) {
         uint32_t logS = ceil_log2(4 * ns);
@@ -462,7 +575,10 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     uint64_t k = (uint64_t)twave * chunk;
     while (k < nr) {
         const uint32_t c = (uint32_t)min<uint64_t>(chunk, nr - k);
-        tail_hits += scan_chunk<HASH, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz);
+        if (PACK && HASH)
+            tail_hits += scan_chunk_packed<PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz, L.pk[wave]);
+        else
+            tail_hits += scan_chunk<HASH, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz);
         if (NW > 1) {
             uint32_t nk = 0;
             if (lane == 0) nk = atomicAdd(&L.next_u, chunk);
@@ -500,7 +616,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     team_sync<NW>();  // table reuse by the next tail
 }
 
-template <bool PLAIN>
+template <bool PLAIN, bool PACK>
 __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) k_tc_main(View<PLAIN> V, TcOut out, TcParams prm,
                                                          uint32_t* __restrict__ q,
                                                          const uint32_t* __restrict__ list,
@@ -526,7 +642,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             const uint32_t i = L.bcast;
             __syncthreads();
             if (i >= cnt) break;
-            team_process<TC_WAVES, true, PLAIN>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.team_chunk,
+            team_process<TC_WAVES, true, PLAIN, PACK>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.team_chunk,
                                                 out, my_total, st_wedge, st_isz);
         }
     }
@@ -551,7 +667,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
                 if (i >= cnt) break;
                 const uint32_t e = min(cnt, i + grab);
                 for (; i < e; ++i)
-                    team_process<1, true, PLAIN>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, KT_WAVE, out,
+                    team_process<1, true, PLAIN, PACK>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, KT_WAVE, out,
                                                  my_total, st_wedge, st_isz);
             }
         }
@@ -600,7 +716,7 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
         const uint32_t i = L.bcast;
         __syncthreads();
         if (i >= cnt) break;
-        team_process<TC_WAVES, false, PLAIN>(V, list[base + i], L, nullptr, nullptr, 0, prm.team_chunk, out,
+        team_process<TC_WAVES, false, PLAIN, false>(V, list[base + i], L, nullptr, nullptr, 0, prm.team_chunk, out,
                                              my_total, st_wedge, st_isz);
     }
     if (lane_id() == 0) {
@@ -645,7 +761,12 @@ static void launch_impl(const TcView& view, const TcOut& out, const TcParams& pr
     // huge tails are rare: a small grid; it exits at once when there are none
     k_tc_huge<PLAIN><<<(unsigned)num_cus, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
     if (ev_begin) hipEventRecord(ev_begin, st);
-    k_tc_main<PLAIN><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
+    TcOut o = out;
+    o.err = stats + ST_ERR;
+    if (p.strategy == TC_PACKED)
+        k_tc_main<PLAIN, true><<<grid, TC_THREADS, 0, st>>>(V, o, p, q, list, stats);
+    else
+        k_tc_main<PLAIN, false><<<grid, TC_THREADS, 0, st>>>(V, o, p, q, list, stats);
     if (ev_end) hipEventRecord(ev_end, st);
 }
 

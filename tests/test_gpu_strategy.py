@@ -1,0 +1,106 @@
+"""Every intersection strategy and bin threshold gives the oracle's counts.
+
+The knobs (kt_params: wave/workgroup bin thresholds, streamed or packed list
+walk, grid size) only move work between kernels and code paths; totals,
+per-vertex counts and the reference's work counters must not change.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import katric_amd as K
+from katric_amd import batch as KB
+from katric_amd import shard as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+STRATEGIES = [K.STRATEGY_STREAMED, K.STRATEGY_PACKED]
+# (wave_max_deg, group_max_deg, grid_per_cu)
+BINS = [(128, 2048, 4), (8, 256, 1), (256, 512, 2), (2, 2048, 3)]
+
+
+@pytest.fixture
+def knobs(ctx):
+    saved = ctx.params()
+
+    def apply(strategy, wave, group, grid):
+        ctx.set_params(wave_max_deg=wave, group_max_deg=group, grid_per_cu=grid, strategy=strategy)
+
+    yield apply
+    ctx.set_params(**saved)
+
+
+def _graphs():
+    yield "rmat14", K.gen_rmat(14, 16, seed=21)
+    yield "gnm", K.gen_gnm(5000, 60000, 22)
+    # one hub joined to a clique: many short T lists (the hub's neighbours)
+    # next to long ones
+    n = 400
+    iu = np.triu_indices(60, 1)
+    clique = np.stack(iu, 1) + 1
+    hub = np.stack([np.zeros(n - 1, np.int64), np.arange(1, n)], 1)
+    ring = np.stack([np.arange(61, n - 1), np.arange(62, n)], 1)
+    yield "hub", K.Graph.from_edges(n, np.concatenate([clique, hub, ring]))
+
+
+GRAPHS = list(_graphs())
+
+
+@pytest.mark.parametrize("strategy,bins", list(itertools.product(STRATEGIES, BINS)))
+def test_plain_counts_every_strategy(ctx, knobs, strategy, bins):
+    knobs(strategy, *bins)
+    for name, g in GRAPHS:
+        total, pv = ctx.count(g)
+        ot, opv, ost = O.run_shmem(g, threads=4)
+        assert total == ot, name
+        np.testing.assert_array_equal(pv, opv, err_msg=name)
+        st = ctx.stats()
+        assert st["intersection_size_local"] == ost["intersection_size_local"], name
+        assert st["wedge_checks"] == ost["wedge_checks"], name
+
+
+@pytest.mark.parametrize("n", [300, 1200])
+def test_packed_complete_graph(ctx, knobs, n):
+    # K_n: lists of every length up to n - 1, several 256-id windows per chunk
+    knobs(K.STRATEGY_PACKED, 128, 2048, 4)
+    iu = np.triu_indices(n, 1)
+    g = K.Graph.from_edges(n, np.stack(iu, 1))
+    t, pv = ctx.count(g)
+    assert t == n * (n - 1) * (n - 2) // 6
+    assert (pv == (n - 1) * (n - 2) // 2).all()
+
+
+def test_packed_batch(ctx, knobs):
+    knobs(K.STRATEGY_PACKED, 64, 1024, 4)
+    b = KB.gen_batch(40, seed=9, n_min=64, n_max=4000)
+    tot, pv = KB.count_batch(ctx, b, per_vertex=True)
+    for g in range(b.num_graphs):
+        t, opv, _ = O.run_shmem(b.graph(g))
+        assert tot[g] == t
+        a, z = int(b.vertex_base[g]), int(b.vertex_base[g + 1])
+        np.testing.assert_array_equal(pv[a:z], opv)
+
+
+@pytest.mark.parametrize("algo", [S.CETRIC, S.DITRIC])
+def test_packed_shard(ctx, knobs, algo):
+    knobs(K.STRATEGY_PACKED, 32, 2048, 4)
+    g = K.gen_rmat(12, 16, seed=5)
+    P = 4
+    total = 0
+    for r in range(P):
+        sh = S.make_shard(g, P, r, algo)
+        loc, t3, pv = S.count_shard(ctx, sh)
+        oloc, ot3, opv, _ = O.rank_sim(g, P, r, algo, threads=4)
+        assert (loc, t3) == (oloc, ot3)
+        full = np.zeros(g.n, np.uint64)
+        full[sh.vertex_ids()] = pv
+        np.testing.assert_array_equal(full, opv)
+        total += loc + t3
+    assert total == O.run_shmem(g)[0]
+
+
+def test_set_params_rejects_bad_strategy(ctx):
+    with pytest.raises(K.KatricError):
+        ctx.set_params(strategy=7)
