@@ -137,6 +137,17 @@ __device__ __forceinline__ uint32_t wave_incl_max_dpp(uint32_t v) {
     return v;
 }
 
+// Sum over the wave (the same DPP pattern as wave_incl_max_dpp, then lane 63).
+__device__ __forceinline__ uint32_t wave_total_dpp(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false); // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false); // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // Multiplicative (Fibonacci) hash into a 2^logS table.
 __device__ __forceinline__ uint32_t hash_slot(uint32_t w, uint32_t logS) {
     return (w * 2654435761u) >> (32 - logS);

@@ -241,14 +241,13 @@ __device__ __forceinline__ void table_insert(uint32_t* keys, uint32_t logB, uint
     }
 }
 
-__device__ __forceinline__ uint32_t match4(const uint4& k, uint32_t w) {
-    return (k.x == w ? 1u : 0u) | (k.y == w ? 2u : 0u) | (k.z == w ? 4u : 0u) | (k.w == w ? 8u : 0u);
-}
-
-// Probes up to 4 ids of this lane (valid bits in vmask); returns the mask of
-// the ids found and bumps each hit slot's w-counter.
+// Probes up to 4 ids of this lane (id e valid iff valid[e]) and returns this
+// lane's number of hits.  Each hit bumps the w-counter of its slot and calls
+// on_hit(e).  The four slot compares of a probe stay lane masks; hits are
+// counted per lane (one add), so the scalar unit only sees the branch masks.
+template <class OnHit>
 __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, uint32_t logB, uint32_t bmask,
-                                           const uint32_t (&w)[4], uint32_t vmask) {
+                                           const uint32_t (&w)[4], const bool (&valid)[4], OnHit on_hit) {
     const uint4* kb = reinterpret_cast<const uint4*>(keys);
     uint32_t bk[4];
     uint4 kk[4];
@@ -257,46 +256,47 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
         bk[e] = bucket_of(w[e], logB);
         kk[e] = kb[bk[e]];
     }
-    uint32_t hits = 0, cont = 0;
+    uint32_t nh = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        const uint32_t m = match4(kk[e], w[e]);
-        const bool valid = (vmask >> e) & 1u;
-        if (valid && m) {
-            atomicAdd(&cnt[4 * bk[e] + __builtin_ctz(m)], 1u);
-            hits |= 1u << e;
+        const bool m0 = kk[e].x == w[e], m1 = kk[e].y == w[e], m2 = kk[e].z == w[e], m3 = kk[e].w == w[e];
+        const bool hit = valid[e] && (m0 || m1 || m2 || m3);
+        nh += hit ? 1u : 0u;
+        if (hit) {
+            atomicAdd(&cnt[4 * bk[e] + (m0 ? 0u : m1 ? 1u : m2 ? 2u : 3u)], 1u);
+            on_hit(e);
         }
-        if (valid && !m && kk[e].w != KT_EMPTY) cont |= 1u << e;
-    }
-    // rare: the home bucket was full and did not hold the id
-    while (__ballot(cont != 0)) {
-        if (cont) {
-            const int e = __builtin_ctz(cont);
-            uint32_t wb = w[0];
-            uint32_t b = bk[0];
-#pragma unroll
-            for (int q = 1; q < 4; ++q)
-                if (e == q) {
-                    wb = w[q];
-                    b = bk[q];
-                }
-            b = (b + 1) & bmask;
-            while (true) {
-                const uint4 k = kb[b];
-                const uint32_t m = match4(k, wb);
-                if (m) {
-                    atomicAdd(&cnt[4 * b + __builtin_ctz(m)], 1u);
-                    hits |= 1u << e;
-                    break;
-                }
-                if (k.w == KT_EMPTY) break;
+        // rare: a full home bucket without w -- w may sit in a later bucket
+        bool go = valid[e] && !hit && kk[e].w != KT_EMPTY;
+        if (__builtin_amdgcn_ballot_w64(go)) {
+            uint32_t b = bk[e];
+            while (go) {
                 b = (b + 1) & bmask;
+                const uint4 k = kb[b];
+                const bool n0 = k.x == w[e], n1 = k.y == w[e], n2 = k.z == w[e], n3 = k.w == w[e];
+                if (n0 || n1 || n2 || n3) {
+                    atomicAdd(&cnt[4 * b + (n0 ? 0u : n1 ? 1u : n2 ? 2u : 3u)], 1u);
+                    on_hit(e);
+                    ++nh;
+                    go = false;
+                } else if (k.w == KT_EMPTY) {
+                    go = false;
+                }
             }
-            cont &= cont - 1;
         }
     }
-    return hits;
+    return nh;
 }
+
+struct NoHitAction {
+    __device__ __forceinline__ void operator()(int) const {}
+};
+
+// 16 bytes at 4-byte alignment (T lists start anywhere); one global_load_dwordx4.
+struct __attribute__((packed, aligned(4))) Ids4 {
+    uint32_t x, y, z, w;
+};
+__device__ __forceinline__ Ids4 load_ids4(const uint32_t* p) { return *reinterpret_cast<const Ids4*>(p); }
 
 // Binary-search probe of the sorted S(x) in global memory (huge tails).
 __device__ __forceinline__ uint32_t probe_bsearch(const uint32_t* __restrict__ s, uint64_t ns, uint64_t* pv,
@@ -337,9 +337,11 @@ struct ProbeCtx {
 // Streams the T lists of up to 64 neighbours R[k0, k0+cnt) through the probe.
 // Lane j first loads u_j and the bounds of T(u_j) (one coalesced load plus two
 // gathers for the whole chunk); the wave then walks the non-empty lists in
-// order, four ids per lane per step, prefetching the next block -- of the
-// same list or the next one -- before probing the current block.  Hits per u
-// are wave-summed and flushed once; returns the chunk's hits (wave-uniform).
+// order, four ids per lane per step (one 16-byte load from the list start,
+// which needs only 4-byte alignment), prefetching the next block -- of the
+// same list or the next one -- before probing the current block.  Positions
+// and list bounds are wave-uniform 32-bit values; hits are counted per lane
+// and summed across the wave once per list.  Returns the chunk's hits.
 template <bool HASH, bool PLAIN>
 __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0, uint32_t cnt, const ProbeCtx& P,
                                                uint64_t ns, uint64_t& st_wedge, uint64_t& st_isz) {
@@ -358,65 +360,65 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
     uint64_t pending = __ballot(lane < cnt && tej > tbj);
     if (pending == 0) return 0;
     const uint32_t* __restrict__ T = V.t_heads();
+    const uint32_t l4 = 4 * lane;
 
     int j = __ffsll((unsigned long long)pending) - 1;
     pending &= pending - 1;
     uint32_t u = readlane32(uj, j);
-    uint64_t tb = readlane64(tbj, j), te = readlane64(tej, j);
-    uint64_t wbase = tb & ~3ull;  // wave block base (uniform)
-    uint4 q = make_uint4(KT_EMPTY, KT_EMPTY, KT_EMPTY, KT_EMPTY);
-    if (wbase + 4 * lane < te) q = *reinterpret_cast<const uint4*>(T + wbase + 4 * lane);
-    uint32_t hits = 0;
+    const uint32_t* base = T + readlane64(tbj, j);  // list start (uniform)
+    // |T(u)| < 2^32: an out-list holds distinct 32-bit ids
+    uint32_t len = (uint32_t)(readlane64(tej, j) - readlane64(tbj, j));
+    uint32_t pos = 0;  // block start within the list (uniform)
+    // Every lane loads (lanes past the list end re-read its start; their ids
+    // are masked by `valid`): an unconditional load keeps the number of
+    // loads in flight static, so the prefetch below is not waited for early.
+    Ids4 q = load_ids4(l4 < len ? base + l4 : base);
+    uint32_t nh = 0;  // this lane's hits on the current list
     uint64_t total = 0;
     while (true) {
         // next block: same list, or the first block of the next non-empty list
-        uint64_t nbase = wbase + 4 * KT_WAVE;
-        uint32_t nu = u;
-        uint64_t ntb = tb, nte = te;
-        bool more = true;
-        bool switch_u = false;
-        if (nbase >= te) {
+        const uint32_t* nbase = base;
+        uint32_t npos = pos + 4 * KT_WAVE, nlen = len, nu = u;
+        bool more = true, switch_u = false;
+        if (npos >= len) {
             switch_u = true;
             if (pending) {
                 const int nj = __ffsll((unsigned long long)pending) - 1;
                 pending &= pending - 1;
                 nu = readlane32(uj, nj);
-                ntb = readlane64(tbj, nj);
-                nte = readlane64(tej, nj);
-                nbase = ntb & ~3ull;
+                const uint64_t ntb = readlane64(tbj, nj);
+                nbase = T + ntb;
+                nlen = (uint32_t)(readlane64(tej, nj) - ntb);
+                npos = 0;
             } else {
                 more = false;
             }
         }
-        uint4 qn = make_uint4(KT_EMPTY, KT_EMPTY, KT_EMPTY, KT_EMPTY);
-        if (more && nbase + 4 * lane < nte) qn = *reinterpret_cast<const uint4*>(T + nbase + 4 * lane);
-        // probe the current block: element e of this lane sits at wbase + 4*lane + e
-        const uint32_t len = (uint32_t)(te - tb);
-        const uint32_t rel0 = (uint32_t)(wbase - tb) + 4 * lane;  // may wrap below zero (-3..-1)
+        const Ids4 qn = load_ids4(npos + l4 < nlen ? nbase + npos + l4 : nbase);
+        // probe the current block: id e of this lane is list entry pos + 4*lane + e
+        const int lim = (int)(len - pos) - (int)l4;
+        const bool valid[4] = {0 < lim, 1 < lim, 2 < lim, 3 < lim};
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-        uint32_t vmask = 0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) vmask |= (rel0 + e < len) ? (1u << e) : 0u;
         if (HASH) {
-            hits += __builtin_popcount(probe4(P.keys, P.cnt, P.logB, P.bmask, w, vmask));
+            nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, valid, NoHitAction{});
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                if ((vmask >> e) & 1u) hits += probe_bsearch(P.s, P.ns, P.pv, w[e]);
+                if (valid[e]) nh += probe_bsearch(P.s, P.ns, P.pv, w[e]);
         }
         if (switch_u) {
-            const uint32_t h = wave_sum(hits);
+            const uint32_t h = wave_total_dpp(nh);
             if (h) {
                 total += h;
                 if (lane == 0 && P.pv) atomic_add_u64(&P.pv[u], h);
             }
-            hits = 0;
+            nh = 0;
         }
         if (!more) break;
         u = nu;
-        tb = ntb;
-        te = nte;
-        wbase = nbase;
+        base = nbase;
+        len = nlen;
+        pos = npos;
         q = qn;
     }
     return total;
@@ -469,7 +471,7 @@ __device__ __forceinline__ uint64_t scan_chunk_packed(const View<PLAIN>& V, uint
     uint32_t carry = 0;  // 1 + the list active at the start of the next window
 
     // resolve the lists of window w0 and load its ids
-    auto window = [&](uint32_t w0, uint32_t (&w)[4], uint32_t (&jj)[4], uint32_t& vmask) {
+    auto window = [&](uint32_t w0, uint32_t (&w)[4], uint32_t (&jj)[4], bool (&ok)[4]) {
         pk.marks[lane] = 0;
         wave_sync();
         if (len != 0 && start >= w0 && start - w0 < 4 * KT_WAVE) {
@@ -478,7 +480,6 @@ __device__ __forceinline__ uint64_t scan_chunk_packed(const View<PLAIN>& V, uint
         }
         wave_sync();
         const uint32_t m4 = pk.marks[lane];
-        vmask = 0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             uint32_t v = wave_incl_max_dpp((m4 >> (8 * e)) & 0xffu);
@@ -492,34 +493,33 @@ __device__ __forceinline__ uint64_t scan_chunk_packed(const View<PLAIN>& V, uint
             const uint32_t ibeg = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)start);
             // v > 0 and ibeg <= pos < iend: list j holds pos, so base_j + pos
             // lies inside T(u_j)
-            const bool ok = v != 0 && pos < total && pos >= ibeg && pos < iend;
+            ok[e] = v != 0 && pos < total && pos >= ibeg && pos < iend;
             jj[e] = v - 1;
             w[e] = KT_EMPTY;
-            if (ok) w[e] = T[(((uint64_t)bhi << 32) | blo) + pos];
-            vmask |= (ok ? 1u : 0u) << e;
+            if (ok[e]) w[e] = T[(((uint64_t)bhi << 32) | blo) + pos];
         }
     };
 
-    uint32_t w[4], jj[4], vm;
-    window(0, w, jj, vm);
+    uint32_t w[4], jj[4];
+    bool ok[4];
+    window(0, w, jj, ok);
+    uint32_t nh = 0;  // this lane's hits
     for (uint32_t w0 = 0; w0 < total; w0 += 4 * KT_WAVE) {
-        uint32_t wn[4] = {KT_EMPTY, KT_EMPTY, KT_EMPTY, KT_EMPTY}, jn[4] = {0, 0, 0, 0}, vn = 0;
-        if (w0 + 4 * KT_WAVE < total) window(w0 + 4 * KT_WAVE, wn, jn, vn);  // prefetch the next window
-        const uint32_t hm = probe4(P.keys, P.cnt, P.logB, P.bmask, w, vm);
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-            if ((hm >> e) & 1u) atomicAdd(&pk.ucnt[jj[e]], 1u);
+        uint32_t wn[4] = {KT_EMPTY, KT_EMPTY, KT_EMPTY, KT_EMPTY}, jn[4] = {0, 0, 0, 0};
+        bool okn[4] = {false, false, false, false};
+        if (w0 + 4 * KT_WAVE < total) window(w0 + 4 * KT_WAVE, wn, jn, okn);  // prefetch the next window
+        nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, ok, [&](int e) { atomicAdd(&pk.ucnt[jj[e]], 1u); });
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             w[e] = wn[e];
             jj[e] = jn[e];
+            ok[e] = okn[e];
         }
-        vm = vn;
     }
     wave_sync();
     const uint32_t h = lane < cnt ? pk.ucnt[lane] : 0u;
     if (h && P.pv) atomic_add_u64(&P.pv[uj], h);
-    return uniform(wave_sum(h));
+    return wave_total_dpp(nh);
 }
 
 // Process tail x with a team of NW waves (NW == 1: this wave alone).
