@@ -76,13 +76,16 @@ typedef struct {
 /* Intersection strategy knobs (bin thresholds and list walk); the package's
  * autotuner (katric_amd.tune) chooses them per graph class.  Counts never
  * depend on them. */
-#define KT_STRATEGY_STREAMED 0u /* one T list at a time, aligned 16-byte blocks (long lists) */
-#define KT_STRATEGY_PACKED 1u   /* a chunk's T lists concatenated into 256-id windows (short lists) */
+#define KT_STRATEGY_STREAMED 0u /* the wave walks one T list at a time, 256 ids per step (long lists) */
+#define KT_STRATEGY_GROUPED 1u  /* groups of 16 lanes walk 4 T lists at a time (short lists) */
+#define KT_ORDER_DEGREE 0u      /* work queues by degree bucket, largest first */
+#define KT_ORDER_VERTEX 1u      /* work queues by vertex id (locality across a batch's graphs) */
 typedef struct {
     uint32_t wave_max_deg;  /* |N+(v)| <= this: one wave per tail, LDS hash (<= 256) */
     uint32_t group_max_deg; /* |N+(v)| <= this: one workgroup per tail, LDS hash (<= 2048) */
     uint32_t grid_per_cu;   /* persistent 512-thread workgroups per compute unit (1..4 fill the CU) */
     uint32_t strategy;      /* KT_STRATEGY_* */
+    uint32_t order;         /* KT_ORDER_* */
 } kt_params;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes bounds the

@@ -42,7 +42,9 @@ __all__ = [
 ]
 
 STRATEGY_STREAMED = 0  # KT_STRATEGY_STREAMED
-STRATEGY_PACKED = 1  # KT_STRATEGY_PACKED
+STRATEGY_GROUPED = 1  # KT_STRATEGY_GROUPED
+ORDER_DEGREE = 0  # KT_ORDER_DEGREE
+ORDER_VERTEX = 1  # KT_ORDER_VERTEX
 KT_DEVICE_INPUT = 0x1
 KT_DEVICE_OUTPUT = 0x2
 
@@ -254,13 +256,14 @@ class Context:
         self.check(lib().kt_get_params(self._h, C.byref(p)), "kt_get_params")
         return {f: getattr(p, f) for f, _ in KtParams._fields_}
 
-    def set_params(self, wave_max_deg=None, group_max_deg=None, grid_per_cu=None, strategy=None, **extra):
+    def set_params(self, wave_max_deg=None, group_max_deg=None, grid_per_cu=None, strategy=None, order=None,
+                   **extra):
         """Intersection knobs (see kt_params); unknown keys are rejected."""
         if extra:
             raise TypeError(f"unknown parameters: {sorted(extra)}")
         cur = self.params()
         new = dict(wave_max_deg=wave_max_deg, group_max_deg=group_max_deg, grid_per_cu=grid_per_cu,
-                   strategy=strategy)
+                   strategy=strategy, order=order)
         cur.update({k: int(v) for k, v in new.items() if v is not None})
         p = KtParams(**cur)
         self.check(lib().kt_set_params(self._h, C.byref(p)), "kt_set_params")

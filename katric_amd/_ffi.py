@@ -48,6 +48,7 @@ class KtParams(C.Structure):
         ("group_max_deg", u32),
         ("grid_per_cu", u32),
         ("strategy", u32),
+        ("order", u32),
     ]
 
 

@@ -151,6 +151,7 @@ extern "C" int kt_create(kt_ctx** out, int device, uint64_t hbm_budget_bytes) {
     c->prm.team_chunk = 16;
     c->prm.grid_per_cu = 4;
     c->prm.strategy = TC_STREAMED;
+    c->prm.order = TC_BY_DEGREE;
     trace("context on device %d: %d CUs, stream %p", device, c->num_cus, (void*)c->stream);
     *out = c;
     return KT_OK;
@@ -190,13 +191,14 @@ extern "C" const char* kt_last_error(const kt_ctx* c) { return c ? c->err : "no 
 extern "C" int kt_set_params(kt_ctx* c, const kt_params* p) {
     if (!c || !p) return KT_EINVAL;
     if (p->wave_max_deg < 2 || p->wave_max_deg > 256 || p->group_max_deg < p->wave_max_deg ||
-        p->group_max_deg > 2048 || p->grid_per_cu < 1 || p->grid_per_cu > 32 || p->strategy > KT_STRATEGY_PACKED)
+        p->group_max_deg > 2048 || p->grid_per_cu < 1 || p->grid_per_cu > 32 || p->strategy > KT_STRATEGY_GROUPED || p->order > KT_ORDER_VERTEX)
         return KT_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
     c->prm.wave_max_deg = p->wave_max_deg;
     c->prm.group_max_deg = p->group_max_deg;
     c->prm.grid_per_cu = p->grid_per_cu;
     c->prm.strategy = p->strategy;
+    c->prm.order = p->order;
     return KT_OK;
 }
 
@@ -206,6 +208,7 @@ extern "C" int kt_get_params(const kt_ctx* c, kt_params* p) {
     p->group_max_deg = c->prm.group_max_deg;
     p->grid_per_cu = c->prm.grid_per_cu;
     p->strategy = c->prm.strategy;
+    p->order = c->prm.order;
     return KT_OK;
 }
 
@@ -430,7 +433,7 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     uint64_t* tstats = tc_stats_ptr(B.ws_tc);
     rc = c->hip_check(hipMemcpyAsync(c->host_small + 1, totals, 8, hipMemcpyDeviceToHost, st), "D2H total");
     if (rc) return rc;
-    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tstats, 24, hipMemcpyDeviceToHost, st), "D2H stats");
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
     if (rc) return rc;
     if (want_pv && !pv_dev) {
         rc = c->hip_check(hipMemcpyAsync(per_vertex_out, pv, uint64_t(n) * 8, hipMemcpyDeviceToHost, st),
@@ -441,7 +444,6 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     rc = c->hip_check(hipStreamSynchronize(st), "count");
     if (rc) return rc;
     trace("count done: total=%llu", (unsigned long long)c->host_small[1]);
-    if (c->host_small[4]) return c->fail(KT_EINVAL, "intersection: out-list stream beyond 32-bit positions");
     *total_out = c->host_small[1];
     c->stats.local_triangles = c->host_small[1];
     c->stats.wedge_checks = c->host_small[2];

@@ -137,6 +137,11 @@ __device__ __forceinline__ uint32_t wave_incl_max_dpp(uint32_t v) {
     return v;
 }
 
+// Maximum over the wave (lane 63 of the inclusive max-scan).
+__device__ __forceinline__ uint32_t wave_max_u32_dpp(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_max_dpp(v), 63);
+}
+
 // Sum over the wave (the same DPP pattern as wave_incl_max_dpp, then lane 63).
 __device__ __forceinline__ uint32_t wave_total_dpp(uint32_t v) {
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);  // row_shr:1

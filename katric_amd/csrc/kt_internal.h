@@ -41,13 +41,19 @@ struct TcParams {
     uint32_t group_max_deg;   // d+(v) <= this: one workgroup (8 waves) per tail vertex (LDS hash)
     uint32_t team_chunk;      // neighbours per work grab inside a workgroup team (1..64)
     uint32_t grid_per_cu;     // persistent workgroups per CU
-    uint32_t strategy;        // TC_STREAMED or TC_PACKED (hash-probe tails)
+    uint32_t strategy;        // TC_STREAMED or TC_GROUPED (hash-probe tails)
+    uint32_t order;           // TC_BY_DEGREE or TC_BY_VERTEX (work-queue order)
 };
 
-// How a team walks the T lists of a chunk of neighbours: one list at a time
-// in aligned 16-byte blocks (TC_STREAMED), or all lists of the chunk
-// concatenated into 256-id windows (TC_PACKED, for short lists).
-enum : uint32_t { TC_STREAMED = 0, TC_PACKED = 1 };
+// How a team walks the T lists of a chunk of neighbours: the whole wave on
+// one list at a time, 256 ids per step (TC_STREAMED, long lists), or groups
+// of 16 lanes on 4 lists at a time (TC_GROUPED, short lists).
+enum : uint32_t { TC_STREAMED = 0, TC_GROUPED = 1 };
+// Order of the work queues: tails by degree bucket, largest first, in 13
+// buckets (TC_BY_DEGREE), or by vertex id in two classes -- workgroup tails,
+// then wave tails (TC_BY_VERTEX: tails of one graph of a batch, and their
+// data, stay together in time and cache).
+enum : uint32_t { TC_BY_DEGREE = 0, TC_BY_VERTEX = 1 };
 
 // Generic "tail list" view for the intersection kernel:
 //   tails x in [0, X): hash set S(x) = sheads[s_off[x] .. s_off[x+1]),
@@ -74,7 +80,6 @@ struct TcView {
 struct TcOut {
     uint64_t* per_vertex;       // nullable
     uint64_t* totals;           // per graph (>= 1 entry)
-    uint64_t* err = nullptr;    // set by tc_launch: the workspace's error word
 };
 
 // Batch pipeline over a disjoint-union CSR (kt_batch.hip), shared with the

@@ -46,10 +46,8 @@ constexpr uint32_t WAVE_GRAB = 8;  // wave-phase tails taken per queue access
 //   Q_PULL + 32 * b       pull cursor of bucket b
 //   Q_FILL + 32 * b       fill cursor of bucket b (binning pass 2)
 enum : int { Q_CNT = 0, Q_PULL = 32, Q_FILL = 32 * 14, Q_STRIDE = 32, Q_WORDS = 32 * 28 };
-// stats words after the queue header: work counters, then an error word
-// (non-zero: a chunk's packed stream exceeded 32-bit positions; the counts of
-// that launch are incomplete)
-enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_ERR = 2, ST_WORDS = 4 };
+// stats words after the queue header (the reference's work counters)
+enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_WORDS = 4 };
 
 template <bool PLAIN>
 struct View {
@@ -114,7 +112,12 @@ __device__ __forceinline__ int tail_bucket(const View<PLAIN>& V, const TcParams&
     r = re - rb;
     if (s == 0 || r == 0 || (PLAIN && s < 2)) return -1;
     if (s > prm.group_max_deg) return BUCKET_HUGE;
-    return (int)ceil_log2(s);  // 0..11
+    const int b = (int)ceil_log2(s);  // 0..11
+    if (prm.order == TC_BY_VERTEX) {  // two classes, each in vertex order
+        const int wb = (int)ceil_log2(prm.wave_max_deg);
+        return b <= wb ? wb : BUCKET_HUGE - 1;
+    }
+    return b;
 }
 
 // Grid-stride over tails; per-block bucket counts in LDS, one global atomic
@@ -198,10 +201,10 @@ __global__ void __launch_bounds__(256) k_tc_bin_fill(View<PLAIN> V, TcParams prm
 // ---------------------------------------------------------------------------
 // the team worker
 // ---------------------------------------------------------------------------
-// Per-wave scratch of the packed scan: start marks of one 256-id window and
-// the hit counters of the (up to 64) neighbours of the current chunk.
+// Per-wave scratch of the grouped scan: the lanes of the chunk's non-empty
+// lists in order, and the hit counters of the chunk's (up to 64) neighbours.
 struct PackLds {
-    uint32_t marks[KT_WAVE];
+    uint32_t marks[KT_WAVE / 4];
     uint32_t ucnt[KT_WAVE];
 };
 
@@ -331,7 +334,6 @@ struct ProbeCtx {
     const uint32_t* s;  // sorted S(x) (binary-search probes)
     uint64_t ns;
     uint64_t* pv;
-    uint64_t* err;
 };
 
 // Streams the T lists of up to 64 neighbours R[k0, k0+cnt) through the probe.
@@ -424,20 +426,22 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
     return total;
 }
 
-// Packed variant of scan_chunk for the hash probe: the T lists of the chunk's
-// neighbours are concatenated into one stream of `total` ids, and the wave
-// walks the stream in windows of 256 ids -- id e * 64 + lane of a window is
-// lane `lane`'s e-th id, so each of the four loads is one contiguous run of
-// 64 ids per list piece.  Short lists therefore share a window instead of
-// taking one each.  The list holding stream position p is the last list
-// starting at or before p: lists mark their start in the window (one byte in
-// LDS), and a max-scan over the window positions carries the latest mark
-// forward.  Hits are attributed to their neighbour u through per-wave LDS
-// counters, flushed with one global atomic per neighbour.
-template <bool PLAIN>
-__device__ __forceinline__ uint64_t scan_chunk_packed(const View<PLAIN>& V, uint64_t k0, uint32_t cnt,
-                                                      const ProbeCtx& P, uint64_t ns, uint64_t& st_wedge,
-                                                      uint64_t& st_isz, PackLds& pk) {
+// Grouped variant of scan_chunk for the hash probe: the wave splits into
+// 64 / G groups of G lanes, and group q streams the chunk's non-empty T lists
+// of rank q, q + 64/G, q + 2*64/G, ... -- four ids per lane per step, G * 4
+// contiguous ids per group (one coalesced run per group), the next block of
+// the same list or the first block of the group's next list prefetched.  A
+// step therefore covers 64/G lists at once: short lists no longer leave most
+// of the wave idle.  Group state (list start, length, position) lives in
+// vector registers, uniform within the group; the next list's start and
+// length are read from the lane that loaded them (ds_bpermute, every lane
+// active).  Hits are added to the neighbour's LDS counter as they occur and
+// flushed with one global atomic per neighbour.
+template <int G, bool PLAIN>
+__device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uint64_t k0, uint32_t cnt,
+                                                       const ProbeCtx& P, uint64_t ns, uint64_t& st_wedge,
+                                                       uint64_t& st_isz, PackLds& pk) {
+    constexpr uint32_t NQ = KT_WAVE / G;
     const uint32_t lane = lane_id();
     uint32_t uj = 0;
     uint64_t tbj = 0, tej = 0;
@@ -445,76 +449,73 @@ __device__ __forceinline__ uint64_t scan_chunk_packed(const View<PLAIN>& V, uint
         uj = V.r_head(k0 + lane);
         V.t_range(uj, tbj, tej);
     }
-    const uint64_t len64 = tej - tbj;
-    const uint64_t incl64 = wave_incl_sum(len64);
-    const uint64_t total64 = uniform64(__builtin_amdgcn_readlane((uint32_t)incl64, KT_WAVE - 1) |
-                                       ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(incl64 >> 32), KT_WAVE - 1)
-                                        << 32));
-    if (total64 >= (1ull << 31)) {
-        // positions below are 32-bit; under the degree order |N+(u)| <=
-        // sqrt(2m), so only inconsistent shard inputs (ghost degrees) get here
-        if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(P.err), 1ull);
-        return 0;
+    {
+        const uint64_t cw = uniform64(wave_sum(tej - tbj));
+        st_wedge += cw;
+        st_isz += cw + (uint64_t)cnt * ns;
     }
-    st_wedge += total64;
-    st_isz += total64 + (uint64_t)cnt * ns;
-    if (total64 == 0) return 0;
-    const uint32_t total = (uint32_t)total64;
-    const uint32_t len = (uint32_t)len64;
-    const uint32_t incl = (uint32_t)incl64;
-    const uint32_t start = incl - len;
-    const uint64_t base = tbj - start;  // stream position p of list j sits at T[base_j + p]
-    const uint32_t base_lo = (uint32_t)base, base_hi = (uint32_t)(base >> 32);
-    const uint32_t* __restrict__ T = V.t_heads();
-    uint8_t* marks8 = reinterpret_cast<uint8_t*>(pk.marks);
+    const bool nonempty = lane < cnt && tej > tbj;
+    const uint64_t nem = __builtin_amdgcn_ballot_w64(nonempty);
+    const uint32_t nne = (uint32_t)__popcll(nem);
+    if (nne == 0) return 0;
+    // order[r] = lane of the r-th non-empty list
+    uint8_t* order = reinterpret_cast<uint8_t*>(pk.marks);
+    if (nonempty) order[__builtin_amdgcn_mbcnt_hi((uint32_t)(nem >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nem, 0))] =
+        (uint8_t)lane;
     pk.ucnt[lane] = 0;
-    uint32_t carry = 0;  // 1 + the list active at the start of the next window
-
-    // resolve the lists of window w0 and load its ids
-    auto window = [&](uint32_t w0, uint32_t (&w)[4], uint32_t (&jj)[4], bool (&ok)[4]) {
-        pk.marks[lane] = 0;
-        wave_sync();
-        if (len != 0 && start >= w0 && start - w0 < 4 * KT_WAVE) {
-            const uint32_t q = start - w0;  // window position q lives at byte (q % 64) * 4 + q / 64
-            marks8[(q & (KT_WAVE - 1)) * 4 + (q >> 6)] = (uint8_t)(lane + 1);
-        }
-        wave_sync();
-        const uint32_t m4 = pk.marks[lane];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            uint32_t v = wave_incl_max_dpp((m4 >> (8 * e)) & 0xffu);
-            v = max(v, carry);
-            carry = uniform(__builtin_amdgcn_readlane(v, KT_WAVE - 1));
-            const uint32_t pos = w0 + e * KT_WAVE + lane;
-            const int src = (int)((v ? v - 1 : 0u) * 4);
-            const uint32_t blo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)base_lo);
-            const uint32_t bhi = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)base_hi);
-            const uint32_t iend = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)incl);
-            const uint32_t ibeg = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)start);
-            // v > 0 and ibeg <= pos < iend: list j holds pos, so base_j + pos
-            // lies inside T(u_j)
-            ok[e] = v != 0 && pos < total && pos >= ibeg && pos < iend;
-            jj[e] = v - 1;
-            w[e] = KT_EMPTY;
-            if (ok[e]) w[e] = T[(((uint64_t)bhi << 32) | blo) + pos];
-        }
+    wave_sync();
+    const uint32_t tb_lo = (uint32_t)tbj, tb_hi = (uint32_t)(tbj >> 32);
+    const uint32_t len32 = (uint32_t)(tej - tbj);  // distinct 32-bit ids: < 2^32
+    const uint32_t* __restrict__ T = V.t_heads();
+    const uint32_t q = lane / G, s4 = 4 * (lane % G);
+    // state of list rank r: source lane, start, length (every lane calls this)
+    auto fetch = [&](uint32_t r, uint32_t& src, const uint32_t*& base, uint32_t& len) {
+        const bool ok = r < nne;
+        src = order[ok ? r : 0];
+        const int a = (int)(src * 4);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)tb_lo);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)tb_hi);
+        const uint32_t l = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)len32);
+        base = T + (((uint64_t)hi << 32) | lo);
+        len = ok ? l : 0u;
     };
-
-    uint32_t w[4], jj[4];
-    bool ok[4];
-    window(0, w, jj, ok);
-    uint32_t nh = 0;  // this lane's hits
-    for (uint32_t w0 = 0; w0 < total; w0 += 4 * KT_WAVE) {
-        uint32_t wn[4] = {KT_EMPTY, KT_EMPTY, KT_EMPTY, KT_EMPTY}, jn[4] = {0, 0, 0, 0};
-        bool okn[4] = {false, false, false, false};
-        if (w0 + 4 * KT_WAVE < total) window(w0 + 4 * KT_WAVE, wn, jn, okn);  // prefetch the next window
-        nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, ok, [&](int e) { atomicAdd(&pk.ucnt[jj[e]], 1u); });
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            w[e] = wn[e];
-            jj[e] = jn[e];
-            ok[e] = okn[e];
+    uint32_t k = q;  // rank of the group's current list
+    uint32_t csrc, nsrc, clen, nlen;
+    const uint32_t *cbase, *nbase;
+    fetch(k, csrc, cbase, clen);
+    fetch(k + NQ, nsrc, nbase, nlen);
+    uint32_t pos = 0;
+    // every lane loads (lanes past their list's end re-read its start, masked
+    // by `valid`): a static number of loads in flight keeps the prefetch from
+    // being waited for early
+    Ids4 cur = load_ids4(s4 < clen ? cbase + s4 : cbase);
+    uint32_t nh = 0;
+    while (__builtin_amdgcn_ballot_w64(clen != 0)) {
+        // the next block of this group: same list, or its next list
+        const bool same = pos + 4 * G < clen;
+        const uint32_t npos = same ? pos + 4 * G : 0u;
+        const uint32_t* bb = same ? cbase : nbase;
+        const uint32_t bl = same ? clen : nlen;
+        const Ids4 nxt = load_ids4(npos + s4 < bl ? bb + npos + s4 : (bl ? bb : cbase));
+        const int lim = (int)clen - (int)(pos + s4);
+        const bool valid[4] = {0 < lim, 1 < lim, 2 < lim, 3 < lim};
+        const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
+        nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, valid, [&](int) { atomicAdd(&pk.ucnt[csrc], 1u); });
+        // advance; a group that switched lists fetches its list after next
+        uint32_t fsrc, flen;
+        const uint32_t* fbase;
+        fetch(same ? k + NQ : k + 2 * NQ, fsrc, fbase, flen);
+        if (!same) {
+            k += NQ;
+            csrc = nsrc;
+            cbase = nbase;
+            clen = nlen;
+            nsrc = fsrc;
+            nbase = fbase;
+            nlen = flen;
         }
+        pos = npos;
+        cur = nxt;
     }
     wave_sync();
     const uint32_t h = lane < cnt ? pk.ucnt[lane] : 0u;
@@ -549,7 +550,6 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     P.s = V.v.sheads + sb;
     P.ns = ns;
     P.pv = out.per_vertex;
-    P.err = out.err;
     uint32_t slots = 0;
     if (HASH) {
         uint32_t logS = ceil_log2(4 * ns);
@@ -576,7 +576,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     while (k < nr) {
         const uint32_t c = (uint32_t)min<uint64_t>(chunk, nr - k);
         if (PACK && HASH)
-            tail_hits += scan_chunk_packed<PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz, L.pk[wave]);
+            tail_hits += scan_chunk_grouped<16, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz, L.pk[wave]);
         else
             tail_hits += scan_chunk<HASH, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz);
         if (NW > 1) {
@@ -656,7 +656,8 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             const uint32_t base = bucket_base(q, b);
             uint32_t* pull = &q[Q_PULL + Q_STRIDE * b];
             // more tails per grab for the small buckets (less queue traffic)
-            const uint32_t grab = b <= 3 ? 4 * WAVE_GRAB : (b <= 5 ? 2 * WAVE_GRAB : WAVE_GRAB);
+            const uint32_t grab =
+                (b <= 3 || prm.order == TC_BY_VERTEX) ? 4 * WAVE_GRAB : (b <= 5 ? 2 * WAVE_GRAB : WAVE_GRAB);
             while (true) {
                 uint32_t i = 0;
                 if (lane == 0)
@@ -761,12 +762,10 @@ static void launch_impl(const TcView& view, const TcOut& out, const TcParams& pr
     // huge tails are rare: a small grid; it exits at once when there are none
     k_tc_huge<PLAIN><<<(unsigned)num_cus, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
     if (ev_begin) hipEventRecord(ev_begin, st);
-    TcOut o = out;
-    o.err = stats + ST_ERR;
-    if (p.strategy == TC_PACKED)
-        k_tc_main<PLAIN, true><<<grid, TC_THREADS, 0, st>>>(V, o, p, q, list, stats);
+    if (p.strategy == TC_GROUPED)
+        k_tc_main<PLAIN, true><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
     else
-        k_tc_main<PLAIN, false><<<grid, TC_THREADS, 0, st>>>(V, o, p, q, list, stats);
+        k_tc_main<PLAIN, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
     if (ev_end) hipEventRecord(ev_end, st);
 }
 

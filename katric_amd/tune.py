@@ -1,7 +1,7 @@
 """Autotuner for the intersection knobs (kt_params).
 
 The knobs -- wave/workgroup bin thresholds, how a team walks its T lists
-(streamed or packed), persistent workgroups per CU -- move work between
+(streamed or grouped), queue order (degree or vertex), persistent workgroups per CU -- move work between
 kernels and code paths and never change a count.  Their best setting depends
 on the degree profile of the input, so the tuner
 

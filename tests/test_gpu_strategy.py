@@ -1,6 +1,6 @@
 """Every intersection strategy and bin threshold gives the oracle's counts.
 
-The knobs (kt_params: wave/workgroup bin thresholds, streamed or packed list
+The knobs (kt_params: wave/workgroup bin thresholds, streamed or grouped list
 walk, grid size) only move work between kernels and code paths; totals,
 per-vertex counts and the reference's work counters must not change.
 """
@@ -16,17 +16,20 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-STRATEGIES = [K.STRATEGY_STREAMED, K.STRATEGY_PACKED]
+STRATEGIES = [K.STRATEGY_STREAMED, K.STRATEGY_GROUPED]
 # (wave_max_deg, group_max_deg, grid_per_cu)
 BINS = [(128, 2048, 4), (8, 256, 1), (256, 512, 2), (2, 2048, 3)]
+
+
+ORDERS = [K.ORDER_DEGREE, K.ORDER_VERTEX]
 
 
 @pytest.fixture
 def knobs(ctx):
     saved = ctx.params()
 
-    def apply(strategy, wave, group, grid):
-        ctx.set_params(wave_max_deg=wave, group_max_deg=group, grid_per_cu=grid, strategy=strategy)
+    def apply(strategy, wave, group, grid, order=K.ORDER_DEGREE):
+        ctx.set_params(wave_max_deg=wave, group_max_deg=group, grid_per_cu=grid, strategy=strategy, order=order)
 
     yield apply
     ctx.set_params(**saved)
@@ -48,9 +51,9 @@ def _graphs():
 GRAPHS = list(_graphs())
 
 
-@pytest.mark.parametrize("strategy,bins", list(itertools.product(STRATEGIES, BINS)))
-def test_plain_counts_every_strategy(ctx, knobs, strategy, bins):
-    knobs(strategy, *bins)
+@pytest.mark.parametrize("strategy,bins,order", list(itertools.product(STRATEGIES, BINS, ORDERS)))
+def test_plain_counts_every_strategy(ctx, knobs, strategy, bins, order):
+    knobs(strategy, *bins, order=order)
     for name, g in GRAPHS:
         total, pv = ctx.count(g)
         ot, opv, ost = O.run_shmem(g, threads=4)
@@ -62,9 +65,9 @@ def test_plain_counts_every_strategy(ctx, knobs, strategy, bins):
 
 
 @pytest.mark.parametrize("n", [300, 1200])
-def test_packed_complete_graph(ctx, knobs, n):
-    # K_n: lists of every length up to n - 1, several 256-id windows per chunk
-    knobs(K.STRATEGY_PACKED, 128, 2048, 4)
+def test_grouped_complete_graph(ctx, knobs, n):
+    # K_n: lists of every length up to n - 1, several blocks per list
+    knobs(K.STRATEGY_GROUPED, 128, 2048, 4)
     iu = np.triu_indices(n, 1)
     g = K.Graph.from_edges(n, np.stack(iu, 1))
     t, pv = ctx.count(g)
@@ -72,8 +75,9 @@ def test_packed_complete_graph(ctx, knobs, n):
     assert (pv == (n - 1) * (n - 2) // 2).all()
 
 
-def test_packed_batch(ctx, knobs):
-    knobs(K.STRATEGY_PACKED, 64, 1024, 4)
+@pytest.mark.parametrize("order", ORDERS)
+def test_grouped_batch(ctx, knobs, order):
+    knobs(K.STRATEGY_GROUPED, 64, 1024, 4, order)
     b = KB.gen_batch(40, seed=9, n_min=64, n_max=4000)
     tot, pv = KB.count_batch(ctx, b, per_vertex=True)
     for g in range(b.num_graphs):
@@ -84,8 +88,8 @@ def test_packed_batch(ctx, knobs):
 
 
 @pytest.mark.parametrize("algo", [S.CETRIC, S.DITRIC])
-def test_packed_shard(ctx, knobs, algo):
-    knobs(K.STRATEGY_PACKED, 32, 2048, 4)
+def test_grouped_shard(ctx, knobs, algo):
+    knobs(K.STRATEGY_GROUPED, 32, 2048, 4)
     g = K.gen_rmat(12, 16, seed=5)
     P = 4
     total = 0
@@ -101,6 +105,8 @@ def test_packed_shard(ctx, knobs, algo):
     assert total == O.run_shmem(g)[0]
 
 
-def test_set_params_rejects_bad_strategy(ctx):
+def test_set_params_rejects_bad_knobs(ctx):
     with pytest.raises(K.KatricError):
         ctx.set_params(strategy=7)
+    with pytest.raises(K.KatricError):
+        ctx.set_params(order=2)
