@@ -35,17 +35,22 @@ constexpr int TC_WAVES = TC_THREADS / KT_WAVE;           // 8
 constexpr uint32_t GROUP_SLOTS = 4096;                   // shared table of the 8-wave team
 constexpr uint32_t WAVE_SLOTS = GROUP_SLOTS / TC_WAVES;  // private table of a 1-wave team (512)
 constexpr uint32_t MIN_LOGS = 6;
-constexpr int NBUCKET = 13;  // bucket b holds tails with |S| in (2^(b-1), 2^b]; bucket 12 = huge
-constexpr int BUCKET_HUGE = 12;
+// Work-queue buckets.  By degree: bucket b < 12 holds tails with |S| in
+// (2^(b-1), 2^b].  By vertex: buckets 0..7 hold the workgroup tails and 8..15
+// the wave tails of the eight contiguous eighths of the tail range, one per
+// XCD, each in vertex order.  Bucket 16 holds the tails too large for LDS.
+constexpr int NBUCKET = 17;
+constexpr int BUCKET_HUGE = 16;
+constexpr int NXCD = 8;  // MI355X: workgroup i runs on XCD i % 8
 constexpr uint32_t WAVE_GRAB = 8;  // wave-phase tails taken per queue access
 
 // work-queue header (u32 words at the start of the workspace); every counter
 // that waves update concurrently sits on its own 128-byte line so that the
 // queues of different buckets land on different L2 channels:
-//   [0, 13)               tails per bucket (written by binning, then read-only)
+//   [0, 17)               tails per bucket (written by binning, then read-only)
 //   Q_PULL + 32 * b       pull cursor of bucket b
 //   Q_FILL + 32 * b       fill cursor of bucket b (binning pass 2)
-enum : int { Q_CNT = 0, Q_PULL = 32, Q_FILL = 32 * 14, Q_STRIDE = 32, Q_WORDS = 32 * 28 };
+enum : int { Q_CNT = 0, Q_PULL = 32, Q_FILL = 32 * 18, Q_STRIDE = 32, Q_WORDS = 32 * 36 };
 // stats words after the queue header (the reference's work counters)
 enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_WORDS = 4 };
 
@@ -113,9 +118,9 @@ __device__ __forceinline__ int tail_bucket(const View<PLAIN>& V, const TcParams&
     if (s == 0 || r == 0 || (PLAIN && s < 2)) return -1;
     if (s > prm.group_max_deg) return BUCKET_HUGE;
     const int b = (int)ceil_log2(s);  // 0..11
-    if (prm.order == TC_BY_VERTEX) {  // two classes, each in vertex order
-        const int wb = (int)ceil_log2(prm.wave_max_deg);
-        return b <= wb ? wb : BUCKET_HUGE - 1;
+    if (prm.order == TC_BY_VERTEX) {  // two classes per XCD, each in vertex order
+        const int xcd = (int)(((uint64_t)x * NXCD) / V.v.X);
+        return b <= (int)ceil_log2(prm.wave_max_deg) ? NXCD + xcd : xcd;
     }
     return b;
 }
@@ -627,8 +632,14 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
     uint64_t my_total = 0, st_wedge = 0, st_isz = 0;
     const int wave_bucket_max = (int)ceil_log2(prm.wave_max_deg);
 
-    // team phase: the workgroup-bin buckets from the largest down
-    for (int b = BUCKET_HUGE - 1; b > wave_bucket_max; --b) {
+    const bool by_vertex = prm.order == TC_BY_VERTEX;
+    const int xcd = (int)(blockIdx.x % NXCD);
+    // team phase: by degree, the workgroup-bin buckets from the largest down;
+    // by vertex, this XCD's workgroup tails first, then the other XCDs'
+    const int team_hi = by_vertex ? NXCD - 1 : BUCKET_HUGE - 1;
+    const int team_lo = by_vertex ? 0 : wave_bucket_max + 1;
+    for (int k = team_hi; k >= team_lo; --k) {
+        const int b = by_vertex ? (xcd + NXCD - 1 - k) % NXCD : k;
         const uint32_t cnt = q[Q_CNT + b];
         if (cnt == 0) continue;
         const uint32_t base = bucket_base(q, b);
@@ -646,18 +657,20 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
                                                 out, my_total, st_wedge, st_isz);
         }
     }
-    // wave phase: independent waves with private tables, largest buckets first
+    // wave phase: independent waves with private tables; by degree, largest
+    // buckets first; by vertex, this XCD's wave tails first
     {
         uint32_t* keys = L.keys + wave * WAVE_SLOTS;
         uint32_t* tcnt = L.cnt + wave * WAVE_SLOTS;
-        for (int b = wave_bucket_max; b >= 0; --b) {
+        const int wave_hi = by_vertex ? NXCD - 1 : wave_bucket_max;
+        for (int k = wave_hi; k >= 0; --k) {
+            const int b = by_vertex ? NXCD + (xcd + NXCD - 1 - k) % NXCD : k;
             const uint32_t cnt = q[Q_CNT + b];
             if (cnt == 0) continue;
             const uint32_t base = bucket_base(q, b);
             uint32_t* pull = &q[Q_PULL + Q_STRIDE * b];
             // more tails per grab for the small buckets (less queue traffic)
-            const uint32_t grab =
-                (b <= 3 || prm.order == TC_BY_VERTEX) ? 4 * WAVE_GRAB : (b <= 5 ? 2 * WAVE_GRAB : WAVE_GRAB);
+            const uint32_t grab = (b <= 3 || by_vertex) ? 4 * WAVE_GRAB : (b <= 5 ? 2 * WAVE_GRAB : WAVE_GRAB);
             while (true) {
                 uint32_t i = 0;
                 if (lane == 0)
