@@ -45,8 +45,6 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # 1.000: 58,503,750 KiB x 2.000 + 4,430,501 KiB = 124.35 GB.
 MEASURED_TRAFFIC = {("plain", 22, 16, 1): 124.35e9}
 
-# library defaults: the one-graph-per-launch baselines run with these
-SINGLE_GRAPH_KNOBS = {"wave_max_deg": 128, "group_max_deg": 2048, "grid_per_cu": 4, "strategy": 0, "order": 0}
 
 
 def parse():
@@ -78,11 +76,16 @@ def parse():
     return ap.parse_args()
 
 
-def apply_params(ctx, args):
-    """Applies --params to the context; returns the knobs in effect."""
+def apply_params(ctx, args, key=None):
+    """Installs the intersection knobs: --params if given, else the autotuned
+    setting of the workload's class (katric_amd/tuned_params.json, nearest
+    class); returns the knobs in effect."""
     if args.params:
         kv = dict(item.split("=") for item in args.params.split(","))
         ctx.set_params(**{k.strip(): int(v) for k, v in kv.items()})
+    elif key is not None:
+        from katric_amd import tune as T
+        T.apply(ctx, key)
     return ctx.params()
 
 
@@ -123,8 +126,9 @@ def run_shard(args):
     mtot = len(s.msg_heads)
     log(f"[bench] shard {args.rank}/{args.ranks}: owned {s.n_local}, local heads {len(s.heads)}, "
         f"ghosts {len(s.ghost_ids)}, messages {len(s.msg_src)} ({mtot} ids), make {time.time() - t0:.1f}s")
+    from katric_amd import tune as T
     ctx = K.Context(0)
-    knobs = apply_params(ctx, args)
+    knobs = apply_params(ctx, args, T.graph_key(g.n, g.offsets))
     ds = S.upload_shard(ctx, s)
     X = s.n_local + len(s.ghost_ids)
     d_pv = ctx.alloc(max(X, 1) * 8)
@@ -264,8 +268,12 @@ def run_serve(args):
     rng = np.random.default_rng(args.seed)
     R = args.requests
     which = rng.integers(0, len(pool), size=R)
+    from katric_amd import tune as T
     ctx = K.Context(0)
-    knobs = apply_params(ctx, args)
+    # device batches of the server mix pool graphs: the class of a 64-graph sample
+    sample = [pool[i] for i in range(min(64, len(pool)))]
+    sample_off = np.concatenate([[0]] + [np.diff(gr.offsets.astype(np.int64)) for gr in sample]).cumsum()
+    knobs = apply_params(ctx, args, T.batch_key(len(sample), sample_off))
     # reference totals, one kt_count per pool graph (also warms the plain path)
     want = np.array([ctx.count(g, per_vertex=False)[0] for g in pool], np.uint64)
 
@@ -367,8 +375,9 @@ def run_batch(args):
     b = KB.gen_batch(args.graphs, seed=args.seed)
     G, N, E = b.num_graphs, b.n_total, b.nnz // 2
     log(f"[bench] batch: {G} graphs, {N} vertices, {E} edges, gen {time.time() - t0:.1f}s")
+    from katric_amd import tune as T
     ctx = K.Context(0)
-    knobs = apply_params(ctx, args)
+    knobs = apply_params(ctx, args, T.batch_key(G, b.offsets))
     db = KB.upload_batch(ctx, b)
     pv = ctx.alloc(max(N, 1) * 8)
     # batched: the first call captures the pipeline, the rest replay it
@@ -398,9 +407,7 @@ def run_batch(args):
     d_hd.upload(loc_heads)
     del loc_off, loc_heads
     single = np.zeros(G, np.uint64)
-    batch_knobs = ctx.params()
-    ctx.set_params(**SINGLE_GRAPH_KNOBS)  # the one-graph path with its own (default) knobs
-    for g in range(min(G, 16)):  # warm the per-graph path
+    for g in range(min(G, 16)):  # warm the per-graph path (same knobs: tuned for these graphs)
         single[g] = ctx.count_device(int(vb[g + 1] - vb[g]), d_off.ptr + 8 * int(vb[g] + g),
                                      d_hd.ptr + 4 * int(off[vb[g]]), pv.ptr + 8 * int(vb[g]))
     ctx.synchronize()
@@ -410,7 +417,6 @@ def run_batch(args):
                                      d_hd.ptr + 4 * int(off[vb[g]]), pv.ptr + 8 * int(vb[g]))
     ctx.synchronize()
     single_s = time.perf_counter() - ts
-    ctx.set_params(**batch_knobs)
     assert np.array_equal(single, tot), "batched totals differ from one-graph-per-launch totals"
     result = {
         "metric": "graphs/s (exact triangle counts, 8192 mixed G(n,m)/RMAT graphs, batched)",
@@ -450,8 +456,9 @@ def run_plain(args):
     log(f"[bench] RMAT-{args.scale} ef{args.edge_factor}: n={n} m={m} gen {time.time() - t0:.1f}s")
     checksum = g.checksum()
 
+    from katric_amd import tune as T
     ctx = K.Context(0)
-    knobs = apply_params(ctx, args)
+    knobs = apply_params(ctx, args, T.graph_key(g.n, g.offsets))
     dg = ctx.upload(g)          # the symmetric CSR resident in HBM
     d_pv = ctx.alloc(n * 8)     # per-vertex uint64 counts, in HBM
     ctx.synchronize()

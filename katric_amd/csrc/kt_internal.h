@@ -41,14 +41,14 @@ struct TcParams {
     uint32_t group_max_deg;   // d+(v) <= this: one workgroup (8 waves) per tail vertex (LDS hash)
     uint32_t team_chunk;      // neighbours per work grab inside a workgroup team (1..64)
     uint32_t grid_per_cu;     // persistent workgroups per CU
-    uint32_t strategy;        // TC_STREAMED or TC_GROUPED (hash-probe tails)
+    uint32_t strategy;        // TC_STREAMED or TC_GROUPED* (hash-probe tails)
     uint32_t order;           // TC_BY_DEGREE or TC_BY_VERTEX (work-queue order)
 };
 
 // How a team walks the T lists of a chunk of neighbours: the whole wave on
 // one list at a time, 256 ids per step (TC_STREAMED, long lists), or groups
-// of 16 lanes on 4 lists at a time (TC_GROUPED, short lists).
-enum : uint32_t { TC_STREAMED = 0, TC_GROUPED = 1 };
+// of 16, 8 or 32 lanes on 4, 8 or 2 lists at a time (TC_GROUPED*, short lists).
+enum : uint32_t { TC_STREAMED = 0, TC_GROUPED16 = 1, TC_GROUPED8 = 2, TC_GROUPED32 = 3 };
 // Order of the work queues: tails by degree bucket, largest first, in 13
 // buckets (TC_BY_DEGREE), or by vertex id in two classes -- workgroup tails,
 // then wave tails (TC_BY_VERTEX: tails of one graph of a batch, and their

@@ -529,7 +529,7 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
 }
 
 // Process tail x with a team of NW waves (NW == 1: this wave alone).
-template <int NW, bool HASH, bool PLAIN, bool PACK>
+template <int NW, bool HASH, bool PLAIN, int GL>
 __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, Lds& L, uint32_t* keys,
                                              uint32_t* cnt, uint32_t cap_slots, uint32_t chunk, const TcOut& out,
                                              uint64_t& my_total, uint64_t& st_wedge, uint64_t& st_isz) {
@@ -580,8 +580,8 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     uint64_t k = (uint64_t)twave * chunk;
     while (k < nr) {
         const uint32_t c = (uint32_t)min<uint64_t>(chunk, nr - k);
-        if (PACK && HASH)
-            tail_hits += scan_chunk_grouped<16, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz, L.pk[wave]);
+        if (GL < KT_WAVE && HASH)
+            tail_hits += scan_chunk_grouped<GL, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz, L.pk[wave]);
         else
             tail_hits += scan_chunk<HASH, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz);
         if (NW > 1) {
@@ -621,7 +621,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     team_sync<NW>();  // table reuse by the next tail
 }
 
-template <bool PLAIN, bool PACK>
+template <bool PLAIN, int GL>
 __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) k_tc_main(View<PLAIN> V, TcOut out, TcParams prm,
                                                          uint32_t* __restrict__ q,
                                                          const uint32_t* __restrict__ list,
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             const uint32_t i = L.bcast;
             __syncthreads();
             if (i >= cnt) break;
-            team_process<TC_WAVES, true, PLAIN, PACK>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.team_chunk,
+            team_process<TC_WAVES, true, PLAIN, GL>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.team_chunk,
                                                 out, my_total, st_wedge, st_isz);
         }
     }
@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
                 if (i >= cnt) break;
                 const uint32_t e = min(cnt, i + grab);
                 for (; i < e; ++i)
-                    team_process<1, true, PLAIN, PACK>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, KT_WAVE, out,
+                    team_process<1, true, PLAIN, GL>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, KT_WAVE, out,
                                                  my_total, st_wedge, st_isz);
             }
         }
@@ -730,7 +730,7 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
         const uint32_t i = L.bcast;
         __syncthreads();
         if (i >= cnt) break;
-        team_process<TC_WAVES, false, PLAIN, false>(V, list[base + i], L, nullptr, nullptr, 0, prm.team_chunk, out,
+        team_process<TC_WAVES, false, PLAIN, KT_WAVE>(V, list[base + i], L, nullptr, nullptr, 0, prm.team_chunk, out,
                                              my_total, st_wedge, st_isz);
     }
     if (lane_id() == 0) {
@@ -775,10 +775,13 @@ static void launch_impl(const TcView& view, const TcOut& out, const TcParams& pr
     // huge tails are rare: a small grid; it exits at once when there are none
     k_tc_huge<PLAIN><<<(unsigned)num_cus, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
     if (ev_begin) hipEventRecord(ev_begin, st);
-    if (p.strategy == TC_GROUPED)
-        k_tc_main<PLAIN, true><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
-    else
-        k_tc_main<PLAIN, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
+    // lanes per list: the whole wave (streamed) or groups of 32, 16 or 8
+    switch (p.strategy) {
+        case TC_GROUPED32: k_tc_main<PLAIN, 32><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+        case TC_GROUPED16: k_tc_main<PLAIN, 16><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+        case TC_GROUPED8: k_tc_main<PLAIN, 8><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+        default: k_tc_main<PLAIN, KT_WAVE><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+    }
     if (ev_end) hipEventRecord(ev_end, st);
 }
 

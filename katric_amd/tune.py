@@ -30,7 +30,7 @@ from typing import Callable, Dict, Iterable, List, Optional
 import numpy as np
 
 PACKAGE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_params.json")
-DEFAULTS = {"wave_max_deg": 128, "group_max_deg": 2048, "grid_per_cu": 4, "strategy": 0}
+DEFAULTS = {"wave_max_deg": 128, "group_max_deg": 2048, "grid_per_cu": 4, "strategy": 0, "order": 0}
 
 
 def user_table_path() -> str:
@@ -58,17 +58,24 @@ def batch_key(num_graphs: int, offsets: np.ndarray) -> str:
     return f"batch/g{_lg(num_graphs)}/d{_lg(avg)}/D{_lg(mx)}"
 
 
-def candidates(quick: bool = False) -> List[Dict[str, int]]:
-    """The search space: every combination the library accepts, pruned to the
-    values that change the binning of realistic graphs."""
-    strategies = [0, 1]
-    waves = [64, 128, 256] if quick else [32, 64, 128, 256]
-    groups = [2048] if quick else [1024, 2048]
-    grids = [4] if quick else [2, 3, 4]
+STAGES = [
+    # stage 1: how lists are walked and in which order tails are taken
+    {"strategy": [0, 1, 2, 3], "order": [0, 1]},
+    # stage 2: the bin thresholds and the grid, under the best stage-1 choice
+    {"wave_max_deg": [32, 64, 128, 256], "group_max_deg": [1024, 2048], "grid_per_cu": [2, 3, 4]},
+]
+
+
+def candidates(stage: dict, base: Dict[str, int]) -> List[Dict[str, int]]:
+    """Every combination of one stage's knobs on top of `base` (combinations
+    the library rejects -- a workgroup bin below the wave bin -- are skipped)."""
+    keys = sorted(stage)
     out = []
-    for s, w, g, c in itertools.product(strategies, waves, groups, grids):
-        if g >= w:
-            out.append({"strategy": s, "wave_max_deg": w, "group_max_deg": g, "grid_per_cu": c})
+    for vals in itertools.product(*(stage[k] for k in keys)):
+        p = dict(base)
+        p.update(zip(keys, vals))
+        if p["group_max_deg"] >= p["wave_max_deg"]:
+            out.append(p)
     return out
 
 
@@ -123,39 +130,44 @@ def apply(ctx, key: str, table: Optional[Dict[str, dict]] = None) -> Dict[str, i
     return p
 
 
-def tune(ctx, run: Callable[[], float], space: Optional[Iterable[Dict[str, int]]] = None, reps: int = 3,
-         key: Optional[str] = None, save: bool = False, path: Optional[str] = None, log=None) -> dict:
-    """Times `run` (which performs one counting call on `ctx` and returns its
-    device milliseconds) under every candidate; returns the best entry
-    {"params", "ms", "default_ms", "trials"} and installs it on `ctx`.  With
-    `save`, the entry is stored under `key` in the user table (or `path`)."""
-    space = list(space) if space is not None else candidates()
+def tune(ctx, run: Callable[[], float], reps: int = 3, key: Optional[str] = None, save: bool = False,
+         path: Optional[str] = None, log=None, stages=None) -> dict:
+    """Times `run` (one counting call on `ctx` returning its device
+    milliseconds) stage by stage: each stage tries every combination of its
+    knobs on top of the best setting so far and keeps the fastest (median of
+    `reps` runs after one warm-up run).  Returns {"params", "ms",
+    "default_ms", "trials"} and installs the winner on `ctx`; with `save`,
+    stores it under `key` in the user table (or `path`)."""
     saved = ctx.params()
-    trials = []
+    best = dict(DEFAULTS)
+    trials = 0
+
+    def measure(p):
+        ctx.set_params(**p)
+        run()  # warm-up (and, for batches, the capture of this setting)
+        return float(np.median([run() for _ in range(reps)]))
+
     try:
-        for p in space:
-            ctx.set_params(**p)
-            run()  # warm-up (and, for batches, the capture of this setting)
-            ms = float(np.median([run() for _ in range(reps)]))
-            trials.append({"params": p, "ms": ms})
-            if log:
-                log(f"[tune] {p} {ms:.3f} ms")
-        ctx.set_params(**DEFAULTS)
-        run()
-        default_ms = float(np.median([run() for _ in range(reps)]))
+        default_ms = best_ms = measure(best)
+        for stage in (stages or STAGES):
+            for p in candidates(stage, best):
+                ms = measure(p)
+                trials += 1
+                if log:
+                    log(f"[tune] {p} {ms:.3f} ms")
+                if ms < best_ms:
+                    best, best_ms = p, ms
     finally:
         ctx.set_params(**saved)
-    best = min(trials, key=lambda t: t["ms"])
-    entry = {"params": best["params"], "ms": best["ms"], "default_ms": default_ms,
-             "trials": len(trials), "measured": time.strftime("%Y-%m-%d")}
-    ctx.set_params(**best["params"])
+    entry = {"params": best, "ms": best_ms, "default_ms": default_ms, "trials": trials,
+             "measured": time.strftime("%Y-%m-%d")}
+    ctx.set_params(**best)
     if save and key:
         save_result(key, entry, path)
     return entry
 
 
-def tune_graph(ctx, g, reps: int = 3, quick: bool = False, save: bool = False, path: Optional[str] = None,
-               log=None) -> dict:
+def tune_graph(ctx, g, reps: int = 3, save: bool = False, path: Optional[str] = None, log=None) -> dict:
     """Tunes kt_count on a graph resident in HBM (the graph is uploaded once)."""
     dg = ctx.upload(g)
     pv = ctx.alloc(max(g.n, 1) * 8)
@@ -165,13 +177,12 @@ def tune_graph(ctx, g, reps: int = 3, quick: bool = False, save: bool = False, p
         return ctx.timing()["count_ms"]
 
     key = graph_key(g.n, g.offsets)
-    e = tune(ctx, run, candidates(quick), reps, key, save, path, log)
+    e = tune(ctx, run, reps, key, save, path, log)
     e["key"] = key
     return e
 
 
-def tune_batch(ctx, b, reps: int = 3, quick: bool = False, save: bool = False, path: Optional[str] = None,
-               log=None) -> dict:
+def tune_batch(ctx, b, reps: int = 3, save: bool = False, path: Optional[str] = None, log=None) -> dict:
     """Tunes kt_batch_count on a batch resident in HBM; each setting is
     captured once and then replayed."""
     from . import batch as KB
@@ -184,6 +195,45 @@ def tune_batch(ctx, b, reps: int = 3, quick: bool = False, save: bool = False, p
         return ctx.timing()["count_ms"]  # the replayed device pipeline
 
     key = batch_key(b.num_graphs, b.offsets)
-    e = tune(ctx, run, candidates(quick), reps, key, save, path, log)
+    e = tune(ctx, run, reps, key, save, path, log)
     e["key"] = key
     return e
+
+
+def main(argv=None) -> None:
+    """Tunes the standard workloads on device 0 and writes the table:
+    python -m katric_amd.tune [--out PATH] [--workloads rmat22,batch,serve]"""
+    import argparse
+    import sys
+
+    from . import Context, gen_gnm, gen_rmat
+    from . import batch as KB
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=PACKAGE_TABLE)
+    ap.add_argument("--workloads", default="rmat22,gnm,batch,serve")
+    ap.add_argument("--reps", type=int, default=3)
+    args = ap.parse_args(argv)
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    ctx = Context(0)
+    for wl in args.workloads.split(","):
+        if wl == "rmat22":  # RMAT scale 22, edge factor 16 (the plain drop-in workload)
+            e = tune_graph(ctx, gen_rmat(22, 16, seed=1), args.reps, True, args.out, log)
+        elif wl == "gnm":  # G(n,m), n = 2^16, 2^20 pairs
+            e = tune_graph(ctx, gen_gnm(1 << 16, 1 << 20, 1), args.reps, True, args.out, log)
+        elif wl == "batch":  # 8192 mixed small graphs (the batched workload)
+            e = tune_batch(ctx, KB.gen_batch(8192, seed=1), args.reps, True, args.out, log)
+        elif wl == "serve":  # a device batch of served requests: 64 graphs, n in [2^10, 2^20]
+            b = KB.gen_batch(64, seed=2, n_min=1 << 10, n_max=1 << 20, deg_min=4.0, deg_max=32.0)
+            e = tune_batch(ctx, b, args.reps, True, args.out, log)
+        else:
+            raise SystemExit(f"unknown workload {wl}")
+        print(json.dumps({"workload": wl, **e}), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

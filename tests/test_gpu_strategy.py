@@ -16,7 +16,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-STRATEGIES = [K.STRATEGY_STREAMED, K.STRATEGY_GROUPED]
+STRATEGIES = [K.STRATEGY_STREAMED, K.STRATEGY_GROUPED16, K.STRATEGY_GROUPED8, K.STRATEGY_GROUPED32]
 # (wave_max_deg, group_max_deg, grid_per_cu)
 BINS = [(128, 2048, 4), (8, 256, 1), (256, 512, 2), (2, 2048, 3)]
 
@@ -65,9 +65,10 @@ def test_plain_counts_every_strategy(ctx, knobs, strategy, bins, order):
 
 
 @pytest.mark.parametrize("n", [300, 1200])
-def test_grouped_complete_graph(ctx, knobs, n):
+@pytest.mark.parametrize("strategy", STRATEGIES[1:])
+def test_grouped_complete_graph(ctx, knobs, n, strategy):
     # K_n: lists of every length up to n - 1, several blocks per list
-    knobs(K.STRATEGY_GROUPED, 128, 2048, 4)
+    knobs(strategy, 128, 2048, 4)
     iu = np.triu_indices(n, 1)
     g = K.Graph.from_edges(n, np.stack(iu, 1))
     t, pv = ctx.count(g)
@@ -76,8 +77,9 @@ def test_grouped_complete_graph(ctx, knobs, n):
 
 
 @pytest.mark.parametrize("order", ORDERS)
-def test_grouped_batch(ctx, knobs, order):
-    knobs(K.STRATEGY_GROUPED, 64, 1024, 4, order)
+@pytest.mark.parametrize("strategy", STRATEGIES[1:])
+def test_grouped_batch(ctx, knobs, order, strategy):
+    knobs(strategy, 64, 1024, 4, order)
     b = KB.gen_batch(40, seed=9, n_min=64, n_max=4000)
     tot, pv = KB.count_batch(ctx, b, per_vertex=True)
     for g in range(b.num_graphs):
@@ -88,8 +90,9 @@ def test_grouped_batch(ctx, knobs, order):
 
 
 @pytest.mark.parametrize("algo", [S.CETRIC, S.DITRIC])
-def test_grouped_shard(ctx, knobs, algo):
-    knobs(K.STRATEGY_GROUPED, 32, 2048, 4)
+@pytest.mark.parametrize("order", ORDERS)
+def test_grouped_shard(ctx, knobs, algo, order):
+    knobs(K.STRATEGY_GROUPED8, 32, 2048, 4, order)
     g = K.gen_rmat(12, 16, seed=5)
     P = 4
     total = 0
