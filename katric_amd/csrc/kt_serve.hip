@@ -32,19 +32,57 @@ static double now_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-// off[vb_g .. vb_g + n_g) += eb_g, heads[eb_g ..) += vb_g, off[N] = nnz
-__global__ void k_serve_rebase(uint32_t G, const uint64_t* __restrict__ vb, const uint64_t* __restrict__ eb,
-                               uint64_t* __restrict__ off, uint32_t* __restrict__ heads) {
-    for (uint32_t g = blockIdx.x; g < G; g += gridDim.x) {
-        const uint64_t v0 = vb[g], v1 = vb[g + 1], e0 = eb[g], e1 = eb[g + 1];
-        for (uint64_t i = v0 + threadIdx.x; i < v1; i += blockDim.x) off[i] += e0;
-        // a head outside its own graph becomes an invalid id (flagged by the orientation pass)
-        for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-            const uint32_t h = heads[e];
-            heads[e] = (h < v1 - v0) ? (uint32_t)(h + v0) : 0xffffffffu;
+// One request of a device batch: where its CSR is read from (page-locked host
+// memory, read by the GPU across PCIe, or the batch arena when the host
+// arrays had to be copied in first), its place in the batch, and its first
+// work chunk.
+struct GatherDesc {
+    const uint64_t* off;
+    const uint32_t* heads;
+    uint64_t n, nnz;
+    uint64_t vb, eb;  // vertex / edge base in the batch
+    uint64_t c0;      // first chunk (offsets chunks, then heads chunks)
+    uint64_t pad;
+};
+constexpr uint32_t GATHER_CHUNK = 4096;  // ids per chunk
+
+__host__ __device__ inline uint64_t gather_chunks(uint64_t n, uint64_t nnz) {
+    return (n + GATHER_CHUNK - 1) / GATHER_CHUNK + (nnz + GATHER_CHUNK - 1) / GATHER_CHUNK;
+}
+
+// Builds the batch's disjoint-union CSR in one launch: off[vb_g + i] =
+// off_g[i] + eb_g, heads[eb_g + k] = heads_g[k] + vb_g (a head outside its
+// own graph becomes an invalid id, flagged by the orientation pass), off[N] =
+// nnz.  Blocks take chunks of one request's offsets or heads; reads are
+// coalesced across the block, so a page-locked source streams at PCIe rate
+// without one copy command per request.
+__global__ void __launch_bounds__(256) k_serve_gather(uint32_t G, const GatherDesc* __restrict__ d,
+                                                      uint64_t chunks, uint64_t* __restrict__ off,
+                                                      uint32_t* __restrict__ heads, uint64_t N, uint64_t nnz) {
+    for (uint64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        uint32_t lo = 0, hi = G - 1;  // last request with c0 <= c
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if (d[mid].c0 <= c)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const GatherDesc r = d[lo];
+        const uint64_t oc = (r.n + GATHER_CHUNK - 1) / GATHER_CHUNK;
+        const uint64_t k = c - r.c0;
+        if (k < oc) {
+            const uint64_t i0 = k * GATHER_CHUNK, i1 = min(r.n, i0 + GATHER_CHUNK);
+            for (uint64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) off[r.vb + i] = r.off[i] + r.eb;
+        } else {
+            const uint64_t e0 = (k - oc) * GATHER_CHUNK, e1 = min(r.nnz, e0 + GATHER_CHUNK);
+            for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+                const uint32_t h = r.heads[e];
+                heads[r.eb + e] = (h < r.n) ? (uint32_t)(h + r.vb) : 0xffffffffu;
+            }
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) off[vb[G]] = eb[G];
+    if (blockIdx.x == 0 && threadIdx.x == 0) off[N] = nnz;
 }
 
 struct ServeReq {
@@ -53,6 +91,8 @@ struct ServeReq {
     uint64_t nnz;
     const uint64_t* off;
     const uint32_t* heads;
+    const uint64_t* d_off = nullptr;    // device-readable aliases of page-locked
+    const uint32_t* d_heads = nullptr;  // host arrays (nullptr: copy them in)
     uint64_t* pv_out;
     double t_submit = 0, t_launch = 0, t_done = 0;
     int status = KT_ENOTREADY;
@@ -70,7 +110,7 @@ struct ServeSlot {
     void* arena = nullptr;
     uint64_t arena_bytes = 0;
     uint64_t* h_small = nullptr;  // pinned, 64 words
-    uint64_t* h_meta = nullptr;   // pinned: vbase, ebase (2 * (cap + 1))
+    uint64_t* h_meta = nullptr;   // pinned: vbase (cap + 1), then the gather descriptors (cap)
     uint64_t* h_totals = nullptr; // pinned, cap
     uint64_t meta_cap = 0;
     std::vector<ServeReq*> reqs;
@@ -122,7 +162,8 @@ static int slot_grow(ServeSlot& s, uint64_t bytes, uint32_t G) {
         s.h_meta = s.h_totals = nullptr;
         s.meta_cap = 0;
         uint64_t cap = std::max<uint64_t>(2 * (uint64_t)G + 2, 64);
-        if (hipHostMalloc(reinterpret_cast<void**>(&s.h_meta), 2 * (cap + 1) * 8, hipHostMallocDefault) != hipSuccess ||
+        if (hipHostMalloc(reinterpret_cast<void**>(&s.h_meta), (cap + 1) * 8 + cap * sizeof(GatherDesc),
+                          hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc(reinterpret_cast<void**>(&s.h_totals), cap * 8, hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
             return KT_ENOMEM;
@@ -142,32 +183,50 @@ static int launch(ServeState* S, ServeSlot& sl, bool sort) {
     }
     BatchBufs B;
     const uint64_t used = (batch_layout(nullptr, G, N, nnz, true, true, sort, B) + 255) & ~255ull;
-    int rc = slot_grow(sl, used + ((uint64_t)G + 1) * 8, G);
+    int rc = slot_grow(sl, used + (uint64_t)G * sizeof(GatherDesc), G);
     if (rc) return rc;
     batch_layout(sl.arena, G, N, nnz, true, true, sort, B);
-    uint64_t* d_eb = reinterpret_cast<uint64_t*>(static_cast<char*>(sl.arena) + used);
+    GatherDesc* d_desc = reinterpret_cast<GatherDesc*>(static_cast<char*>(sl.arena) + used);
     uint64_t* vb = sl.h_meta;
-    uint64_t* eb = sl.h_meta + (G + 1);
-    vb[0] = eb[0] = 0;
-    for (uint32_t g = 0; g < G; ++g) {
-        vb[g + 1] = vb[g] + sl.reqs[g]->n;
-        eb[g + 1] = eb[g] + sl.reqs[g]->nnz;
-    }
+    GatherDesc* desc = reinterpret_cast<GatherDesc*>(sl.h_meta + (sl.meta_cap + 1));
     hipStream_t st = sl.stream;
     uint64_t* d_off = const_cast<uint64_t*>(B.off);
     uint32_t* d_heads = const_cast<uint32_t*>(B.heads);
     const double t = now_ms();
+    vb[0] = 0;
+    uint64_t e = 0, chunks = 0;
     for (uint32_t g = 0; g < G; ++g) {
         ServeReq* r = sl.reqs[g];
         r->t_launch = t;
-        if (r->n && hipMemcpyAsync(d_off + vb[g], r->off, (uint64_t)r->n * 8, hipMemcpyHostToDevice, st) != hipSuccess)
-            return KT_EHIP;
-        if (r->nnz && hipMemcpyAsync(d_heads + eb[g], r->heads, r->nnz * 4, hipMemcpyHostToDevice, st) != hipSuccess)
-            return KT_EHIP;
+        GatherDesc& dg = desc[g];
+        dg.n = r->n;
+        dg.nnz = r->nnz;
+        dg.vb = vb[g];
+        dg.eb = e;
+        dg.c0 = chunks;
+        dg.off = r->d_off;
+        dg.heads = r->d_heads;
+        if (!r->d_off) {  // pageable host memory: copy in, rebase in place
+            if (r->n && hipMemcpyAsync(d_off + vb[g], r->off, (uint64_t)r->n * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+                return KT_EHIP;
+            dg.off = d_off + vb[g];
+        }
+        if (!r->d_heads) {
+            if (r->nnz && hipMemcpyAsync(d_heads + e, r->heads, r->nnz * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+                return KT_EHIP;
+            dg.heads = d_heads + e;
+        }
+        chunks += gather_chunks(r->n, r->nnz);
+        vb[g + 1] = vb[g] + r->n;
+        e += r->nnz;
     }
     if (hipMemcpyAsync(B.vbase, vb, ((uint64_t)G + 1) * 8, hipMemcpyHostToDevice, st) != hipSuccess) return KT_EHIP;
-    if (hipMemcpyAsync(d_eb, eb, ((uint64_t)G + 1) * 8, hipMemcpyHostToDevice, st) != hipSuccess) return KT_EHIP;
-    k_serve_rebase<<<std::min<uint32_t>(G, 4096), 256, 0, st>>>(G, B.vbase, d_eb, d_off, d_heads);
+    if (hipMemcpyAsync(d_desc, desc, (uint64_t)G * sizeof(GatherDesc), hipMemcpyHostToDevice, st) != hipSuccess)
+        return KT_EHIP;
+    // a grid of half the CUs: enough reads in flight to stream across PCIe,
+    // while the other slot's batch keeps counting on the rest of the device
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, (uint64_t)S->num_cus / 2));
+    k_serve_gather<<<grid, 256, 0, st>>>(G, d_desc, chunks, d_off, d_heads, N, nnz);
     batch_pipeline(st, S->num_cus, S->prm, B, G, N, nnz, B.pv, sl.h_totals, sl.h_small, sort);
     for (uint32_t g = 0; g < G; ++g) {
         ServeReq* r = sl.reqs[g];
@@ -402,6 +461,22 @@ extern "C" int kt_serve_stop(kt_ctx* c) {
     return serve_stop_impl(c);
 }
 
+// The device address of page-locked host memory (the GPU reads it across
+// PCIe), or nullptr for pageable memory, which is copied in instead.
+template <class T>
+static const T* device_alias(const T* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (a.type != hipMemoryTypeHost || !a.devicePointer) return nullptr;
+    const char* host = a.hostPointer ? static_cast<const char*>(a.hostPointer) : reinterpret_cast<const char*>(p);
+    const char* q = reinterpret_cast<const char*>(p);
+    if (q < host) return nullptr;
+    return reinterpret_cast<const T*>(static_cast<const char*>(a.devicePointer) + (q - host));
+}
+
 extern "C" int kt_serve_submit(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads,
                                uint64_t* per_vertex_out, uint64_t* ticket) {
     if (!c || !ticket || !offsets) return KT_EINVAL;
@@ -415,6 +490,8 @@ extern "C" int kt_serve_submit(kt_ctx* c, uint32_t n, const uint64_t* offsets, c
     r->nnz = nnz;
     r->off = offsets;
     r->heads = heads;
+    r->d_off = device_alias(offsets);
+    r->d_heads = nnz ? device_alias(heads) : nullptr;
     r->pv_out = per_vertex_out;
     r->t_submit = now_ms();
     {

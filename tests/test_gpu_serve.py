@@ -105,3 +105,45 @@ def test_serve_cancel_and_unsorted(ctx):
         srv.stop()
     assert all(r.total == want and r.status == 0 for r in res[:-1])
     assert res[-1].status in (0, K.KT_ECANCELED)
+
+
+def test_serve_page_locked_and_pageable_mix(ctx):
+    # page-locked requests are read by the device across PCIe (one gather
+    # launch per batch); pageable ones are copied in first -- results agree
+    graphs = mix(30, 5)
+    alone = [ctx.count(g) for g in graphs]
+    with K.Context(0) as sctx:
+        srv = KS.Server(sctx)
+        keep, tickets, pvs = [], [], []
+        for i, g in enumerate(graphs):
+            if i % 3:
+                view, own = KS.pinned_graph(g)
+                keep += own
+                g = view
+            pv = np.zeros(g.n, np.uint64)
+            pvs.append(pv)
+            tickets.append(srv.submit(g, pv))
+        res = [srv.wait(t) for t in tickets]
+        srv.stop()
+    for (t, pv), r, p in zip(alone, res, pvs):
+        assert r.status == 0 and r.total == t
+        np.testing.assert_array_equal(p, pv)
+
+
+def test_serve_page_locked_malformed_isolated(ctx):
+    graphs = mix(6, 6)
+    out_of_range = K.Graph(2, np.array([0, 1, 2], np.uint64), np.array([1, 7], np.uint32))
+    alone = [ctx.count(g, per_vertex=False)[0] for g in graphs]
+    keep = []
+    with K.Context(0) as sctx:
+        srv = KS.Server(sctx)
+        tk = []
+        for g in graphs[:3] + [out_of_range] + graphs[3:]:
+            view, own = KS.pinned_graph(g)
+            keep += own
+            tk.append(srv.submit(view))
+        res = [srv.wait(t) for t in tk]
+        srv.stop()
+    good = res[:3] + res[4:]
+    assert [r.total for r in good] == alone
+    assert res[3].status == K.KT_EINVAL
