@@ -36,11 +36,17 @@ constexpr int SH_THREADS = 256;
 constexpr int SH_ITEMS = 16;
 constexpr int SH_TILE = SH_THREADS * SH_ITEMS;
 
+// Global id -> combined index (0xffffffff: unknown to this rank).  Ghosts are
+// looked up in a dense table over all global ids when it fits (one gather),
+// else by binary search in the sorted ghost ids.
 struct ShardMap {
     uint32_t from, to, nl, ng;
     const uint32_t* ghost_ids;
+    const uint32_t* dense;  // n_all entries, or nullptr
+    uint32_t n_all;
     __device__ __forceinline__ uint32_t map(uint32_t u) const {
         if (u >= from && u < to) return u - from;
+        if (dense) return u < n_all ? dense[u] : 0xffffffffu;
         uint32_t lo = 0, hi = ng;
         while (lo < hi) {
             uint32_t mid = (lo + hi) >> 1;
@@ -75,6 +81,18 @@ __global__ void k_shard_prepare(ShardMap M, uint64_t nnz, const uint64_t* __rest
             uint64_t d = ghost_degree[g];
             deg[c] = d > 0xffffffffull ? 0xffffffffu : (uint32_t)d;
         }
+    }
+}
+
+// dense[ghost_ids[g]] = nl + g (the table is pre-filled with 0xffffffff)
+__global__ void k_dense_ghosts(uint32_t nl, uint32_t ng, const uint32_t* __restrict__ ghost_ids, uint32_t n_all,
+                               uint32_t* __restrict__ dense, uint32_t* __restrict__ err) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+        const uint32_t id = ghost_ids[g];
+        if (id < n_all)
+            dense[id] = nl + g;
+        else
+            atomicOr(err, KT_ERR_GHOST);
     }
 }
 
@@ -257,7 +275,11 @@ struct ShardBufs {
     uint64_t* scan_tmp;
     uint64_t* tc_ws;
     uint64_t* pv;
+    uint32_t* dense;
 };
+
+// A dense global-id table is used up to 2^30 ids (4 GiB).
+constexpr uint64_t DENSE_MAX_IDS = 1ull << 30;
 
 uint64_t shard_layout(void* base, const kt_shard_in* in, uint64_t nnz, uint64_t mtot, bool host_in, bool pv_dev,
                       ShardBufs& B) {
@@ -290,6 +312,8 @@ uint64_t shard_layout(void* base, const kt_shard_in* in, uint64_t nnz, uint64_t 
     B.scan_tmp = static_cast<uint64_t*>(cv.take(scan_tmp_words((uint64_t)in->n_msg + 1) * 8));
     B.tc_ws = static_cast<uint64_t*>(cv.take(tc_ws_words((uint32_t)std::max<uint64_t>(X, in->n_msg)) * 8));
     B.pv = pv_dev ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(X, 1) * 8)) : nullptr;
+    const uint64_t n_all = in->range_bounds[in->num_ranks];
+    B.dense = n_all <= DENSE_MAX_IDS ? static_cast<uint32_t*>(cv.take(std::max<uint64_t>(n_all, 1) * 4)) : nullptr;
     return cv.used;
 }
 
@@ -373,7 +397,14 @@ extern "C" int kt_count_shard(kt_ctx* c, const kt_shard_in* in, kt_shard_out* ou
     uint32_t* err = reinterpret_cast<uint32_t*>(B.small);
     uint64_t* nkeys = B.small + 1;
     fill_u64(B.small, 64, 0, st);
-    ShardMap M{from, to, nl, ng, B.ghost_ids};
+    const uint64_t n_all = bd[P];
+    ShardMap M{from, to, nl, ng, B.ghost_ids, B.dense, (uint32_t)std::min<uint64_t>(n_all, 0xffffffffull)};
+    if (B.dense) {
+        fill_u32(B.dense, n_all, 0xffffffffu, st);
+        if (ng)
+            k_dense_ghosts<<<(unsigned)std::min<uint64_t>((ng + 255) / 256, 8192), 256, 0, st>>>(nl, ng, B.ghost_ids,
+                                                                                               M.n_all, B.dense, err);
+    }
     const int bits = std::max(1, bits_for(X));  // X itself is never a valid index: keys stay below 2^(2 bits)
     const bool cetric = in->algorithm == KT_SHARD_CETRIC;
     k_shard_prepare<<<(unsigned)std::min<uint64_t>((X + 255) / 256, 8192), 256, 0, st>>>(M, nnz, B.off,
