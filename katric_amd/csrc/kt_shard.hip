@@ -148,18 +148,32 @@ __global__ void __launch_bounds__(SH_THREADS)
     if (flags) atomicOr(err, flags);
 }
 
-// Sorted keys -> CSR over X tails (offsets by boundary detection).
-__global__ void k_keys_csr(uint64_t M, uint32_t X, int bits, const uint64_t* __restrict__ keys,
-                           uint64_t* __restrict__ coff, uint32_t* __restrict__ cheads, uint32_t* __restrict__ err) {
+// Sorted keys -> heads of the CSR over X tails, and the list length of every
+// tail in cnt[0, X) (zeroed by the caller; an exclusive scan makes it the
+// offsets).  Runs of equal tails inside a wave add their length with one
+// atomic from the run's last lane.
+__global__ void k_keys_csr(uint64_t M, int bits, const uint64_t* __restrict__ keys, uint64_t* __restrict__ cnt,
+                           uint32_t* __restrict__ cheads, uint32_t* __restrict__ err) {
     const uint64_t mask = (1ull << bits) - 1;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= M; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t t = i < M ? (keys[i] >> bits) : X;
-        const int64_t tp = i == 0 ? -1 : (int64_t)(keys[i - 1] >> bits);
-        if (i < M) {
-            cheads[i] = (uint32_t)(keys[i] & mask);
-            if (i > 0 && keys[i - 1] == keys[i]) atomicOr(err, KT_ERR_DUPLICATE);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < M; i0 += stride) {  // wave-uniform trip count
+        const uint64_t i = i0 + threadIdx.x;
+        const bool in = i < M;
+        const uint64_t k = in ? keys[i] : ~0ull;
+        const uint64_t t = k >> bits;
+        if (in) {
+            cheads[i] = (uint32_t)(k & mask);
+            if (i > 0 && keys[i - 1] == k) atomicOr(err, KT_ERR_DUPLICATE);
         }
-        for (int64_t r = tp + 1; r <= (int64_t)t; ++r) coff[r] = i;
+        const uint64_t tn = __shfl_down(t, 1, 64);
+        const unsigned lane = lane_id();
+        const bool end = in && (lane == KT_WAVE - 1 || i + 1 == M || tn != t);
+        const uint64_t ends = __ballot(end);
+        if (end) {
+            const uint64_t below = ends & lanemask_lt();
+            const int prev = below ? 63 - __clzll((long long)below) : -1;
+            atomic_add_u64(&cnt[t], (uint64_t)((int)lane - prev));
+        }
     }
 }
 
@@ -309,7 +323,7 @@ uint64_t shard_layout(void* base, const kt_shard_in* in, uint64_t nnz, uint64_t 
     B.mtail = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 4));
     B.msplit = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 8));
     B.mheads2 = static_cast<uint32_t*>(cv.take((mtot + 4) * 4));
-    B.scan_tmp = static_cast<uint64_t*>(cv.take(scan_tmp_words((uint64_t)in->n_msg + 1) * 8));
+    B.scan_tmp = static_cast<uint64_t*>(cv.take(scan_tmp_words(std::max<uint64_t>(X, in->n_msg) + 1) * 8));
     B.tc_ws = static_cast<uint64_t*>(cv.take(tc_ws_words((uint32_t)std::max<uint64_t>(X, in->n_msg)) * 8));
     B.pv = pv_dev ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(X, 1) * 8)) : nullptr;
     const uint64_t n_all = in->range_bounds[in->num_ranks];
@@ -424,8 +438,11 @@ extern "C" int kt_count_shard(kt_ctx* c, const kt_shard_in* in, kt_shard_out* ou
                        (flags & KT_ERR_GHOST) ? " ghost table" : "", (flags & KT_ERR_SELF_LOOP) ? " self-loop" : "",
                        (flags & KT_ERR_DUPLICATE) ? " duplicate" : "");
     radix_sort_u64(B.keys, B.alt, Mk, 2 * bits, B.radix_tmp, st);
-    k_keys_csr<<<(unsigned)std::min<uint64_t>((Mk + 256) / 256, 16384), 256, 0, st>>>(Mk, (uint32_t)X, bits, B.keys,
-                                                                                     B.coff, B.cheads, err);
+    fill_u64(B.coff, X + 1, 0, st);
+    if (Mk)
+        k_keys_csr<<<(unsigned)std::min<uint64_t>((Mk + 255) / 256, 16384), 256, 0, st>>>(Mk, bits, B.keys, B.coff,
+                                                                                         B.cheads, err);
+    scan_u64(B.coff, B.coff, X, B.scan_tmp, st);
     if (nl) k_cut_begin<<<(nl + 255) / 256, 256, 0, st>>>(nl, B.coff, B.cheads, B.cut);
     // messages
     if (in->n_msg) {
