@@ -153,9 +153,4 @@ __device__ __forceinline__ uint32_t wave_total_dpp(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// Multiplicative (Fibonacci) hash into a 2^logS table.
-__device__ __forceinline__ uint32_t hash_slot(uint32_t w, uint32_t logS) {
-    return (w * 2654435761u) >> (32 - logS);
-}
-
 }  // namespace kt

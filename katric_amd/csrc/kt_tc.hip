@@ -235,10 +235,16 @@ __device__ __forceinline__ void team_sync() {
 // hash(w) or, when that was full at insertion, in a later bucket (linear
 // probing over buckets).  A lookup therefore ends at the first bucket that
 // holds w or has an empty last slot.
-__device__ __forceinline__ uint32_t bucket_of(uint32_t w, uint32_t logB) { return hash_slot(w, logB); }
+// Bucket of id w: w xor w rotated right by 12, masked -- three full-rate
+// instructions (a multiplicative hash costs a quarter-rate multiply).  The
+// rotation folds the high bits onto the low ones, so ids that share their low
+// bits still spread; on vertex ids it fills buckets like a multiplicative hash.
+__device__ __forceinline__ uint32_t bucket_of(uint32_t w, uint32_t bmask) {
+    return (w ^ __builtin_amdgcn_alignbit(w, w, 12)) & bmask;
+}
 
 __device__ __forceinline__ void table_insert(uint32_t* keys, uint32_t logB, uint32_t bmask, uint32_t w) {
-    uint32_t b = bucket_of(w, logB);
+    uint32_t b = bucket_of(w, bmask);
     while (true) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -261,7 +267,7 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
     uint4 kk[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        bk[e] = bucket_of(w[e], logB);
+        bk[e] = bucket_of(w[e], bmask);
         kk[e] = kb[bk[e]];
     }
     uint32_t nh = 0;
@@ -275,10 +281,10 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
             on_hit(e);
         }
         // rare: a full home bucket without w -- w may sit in a later bucket
-        bool go = valid[e] && !hit && kk[e].w != KT_EMPTY;
-        if (__builtin_amdgcn_ballot_w64(go)) {
+        // (a divergent branch, skipped as a whole when no lane takes it)
+        if (valid[e] && !hit && kk[e].w != KT_EMPTY) {
             uint32_t b = bk[e];
-            while (go) {
+            while (true) {
                 b = (b + 1) & bmask;
                 const uint4 k = kb[b];
                 const bool n0 = k.x == w[e], n1 = k.y == w[e], n2 = k.z == w[e], n3 = k.w == w[e];
@@ -286,10 +292,9 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
                     atomicAdd(&cnt[4 * b + (n0 ? 0u : n1 ? 1u : n2 ? 2u : 3u)], 1u);
                     on_hit(e);
                     ++nh;
-                    go = false;
-                } else if (k.w == KT_EMPTY) {
-                    go = false;
+                    break;
                 }
+                if (k.w == KT_EMPTY) break;
             }
         }
     }
@@ -379,7 +384,8 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
     // Every lane loads (lanes past the list end re-read its start; their ids
     // are masked by `valid`): an unconditional load keeps the number of
     // loads in flight static, so the prefetch below is not waited for early.
-    Ids4 q = load_ids4(l4 < len ? base + l4 : base);
+    // Addresses are a wave-uniform base plus a 32-bit lane offset.
+    Ids4 q = load_ids4(base + (l4 < len ? l4 : 0u));
     uint32_t nh = 0;  // this lane's hits on the current list
     uint64_t total = 0;
     while (true) {
@@ -401,7 +407,7 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
                 more = false;
             }
         }
-        const Ids4 qn = load_ids4(npos + l4 < nlen ? nbase + npos + l4 : nbase);
+        const Ids4 qn = load_ids4(nbase + (npos + l4 < nlen ? npos + l4 : 0u));
         // probe the current block: id e of this lane is list entry pos + 4*lane + e
         const int lim = (int)(len - pos) - (int)l4;
         const bool valid[4] = {0 < lim, 1 < lim, 2 < lim, 3 < lim};
