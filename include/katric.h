@@ -76,8 +76,10 @@ typedef struct {
 /* Intersection strategy knobs (bin thresholds and list walk); the package's
  * autotuner (katric_amd.tune) chooses them per graph class.  Counts never
  * depend on them. */
-#define KT_STRATEGY_STREAMED 0u /* the wave walks one T list at a time, 256 ids per step (long lists) */
-#define KT_STRATEGY_GROUPED 1u  /* groups of 16 lanes walk 4 T lists at a time (short lists) */
+#define KT_STRATEGY_STREAMED 0u  /* the wave walks one T list at a time, 256 ids per step (long lists) */
+#define KT_STRATEGY_GROUPED16 1u /* groups of 16 lanes walk 4 T lists at a time (short lists) */
+#define KT_STRATEGY_GROUPED8 2u  /* groups of 8 lanes walk 8 T lists at a time (very short lists) */
+#define KT_STRATEGY_GROUPED32 3u /* groups of 32 lanes walk 2 T lists at a time */
 #define KT_ORDER_DEGREE 0u      /* work queues by degree bucket, largest first */
 #define KT_ORDER_VERTEX 1u      /* work queues by vertex id (locality across a batch's graphs) */
 typedef struct {
