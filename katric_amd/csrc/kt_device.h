@@ -84,16 +84,26 @@ __device__ __forceinline__ uint32_t wave_incl_max_u32(uint32_t x) {
 // n rows: afterwards marks[i] = row(tile0 + i) - lo for i < tend - tile0.
 // Rows starting strictly inside the tile mark their first slot (the largest
 // row wins when empty rows share a start), then an inclusive max-scan over
-// the THREADS * ITEMS marks fills the gaps.  Block-wide: every thread calls it.
+// the THREADS * ITEMS marks fills the gaps.  The rows of the tile's first and
+// last slot come from `given` (two entries per tile, from k_tile_bounds) or,
+// when it is null, from two binary searches over the offsets.  Block-wide:
+// every thread calls it.
 template <int THREADS, int ITEMS>
 __device__ __forceinline__ void tile_rows(const uint64_t* __restrict__ off, uint32_t n, uint64_t tile0, uint64_t tend,
                                           uint32_t* marks, uint32_t* bnd, uint32_t* wmax, uint32_t& lo,
-                                          uint32_t& hi) {
+                                          uint32_t& hi, const uint32_t* __restrict__ given = nullptr) {
     constexpr int TILE = THREADS * ITEMS;
     const unsigned tid = threadIdx.x;
     if (tid == 0) {
-        bnd[0] = row_of_slot(off, 0, n - 1, tile0);
-        bnd[1] = row_of_slot(off, 0, n - 1, tend - 1);
+        if (given) {  // clamped: offsets that fail validation must not send the scan out of range
+            const uint64_t t = tile0 / TILE;
+            const uint32_t a = min(given[2 * t], n - 1), b = min(given[2 * t + 1], n - 1);
+            bnd[0] = a;
+            bnd[1] = max(a, b);
+        } else {
+            bnd[0] = row_of_slot(off, 0, n - 1, tile0);
+            bnd[1] = row_of_slot(off, 0, n - 1, tend - 1);
+        }
     }
     for (int i = tid; i < TILE; i += THREADS) marks[i] = 0;
     __syncthreads();

@@ -145,9 +145,25 @@ __device__ __forceinline__ bool keep_edge(uint32_t v, uint32_t dv, uint32_t u, u
     return dv < du || (dv == du && v < u);
 }
 
+// bnd[2t] / bnd[2t+1] = the rows holding the first / last slot of tile t
+// (tiles of `tile` slots), written by the row that holds them, so every tile
+// resolves its rows without a dependent search (bnd is pre-filled with 0).
+__global__ void k_tile_bounds(uint32_t n, uint64_t nnz, uint64_t tile, const uint64_t* __restrict__ off,
+                              uint32_t* __restrict__ bnd) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint64_t a = off[v], b = min(off[v + 1], nnz);
+        if (b <= a) continue;  // empty row (or offsets the validation will reject)
+        for (uint64_t t = a / tile; t * tile < b; ++t) {
+            if (t * tile >= a) bnd[2 * t] = v;
+            const uint64_t last = min((t + 1) * tile, nnz) - 1;
+            if (last >= a && last < b) bnd[2 * t + 1] = v;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(OR_THREADS)
     k_orient_keep(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
-                  const uint32_t* __restrict__ deg, uint64_t* __restrict__ keep_bits,
+                  const uint32_t* __restrict__ deg, const uint32_t* __restrict__ tbnd, uint64_t* __restrict__ keep_bits,
                   uint64_t* __restrict__ dplus, uint64_t* __restrict__ tile_cnt, uint32_t* __restrict__ err) {
     __shared__ uint32_t marks[OR_TILE];
     __shared__ uint32_t rowcnt[OR_TILE];
@@ -159,7 +175,7 @@ __global__ void __launch_bounds__(OR_THREADS)
     const unsigned tid = threadIdx.x;
     for (int i = tid; i < OR_TILE; i += OR_THREADS) rowcnt[i] = 0;
     uint32_t lo, hi;
-    tile_rows<OR_THREADS, OR_ITEMS>(off, n, tile0, tend, marks, bnd, wmax, lo, hi);
+    tile_rows<OR_THREADS, OR_ITEMS>(off, n, tile0, tend, marks, bnd, wmax, lo, hi, tbnd);
     const uint64_t nrows = (uint64_t)hi - lo + 1;
     const bool lds_rows = nrows <= OR_TILE;
     uint32_t flags = 0, cnt = 0;
@@ -372,6 +388,7 @@ struct OrientWs {
     uint64_t* tile_off;  // tiles + 1 (tile_off[tiles] = kept total)
     uint64_t* bits;      // ceil(nnz / 64) + 1
     uint64_t* dplus;     // n + 1
+    uint32_t* tbnd;      // 2 * tiles: rows of each tile's first and last slot
     uint64_t* stmp;
 };
 
@@ -382,13 +399,14 @@ static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
     w.tile_off = w.tile_cnt + w.tiles + 1;
     w.bits = w.tile_off + w.tiles + 1;
     w.dplus = w.bits + (nnz + 63) / 64 + 1;
-    w.stmp = w.dplus + (uint64_t)n + 1;
+    w.tbnd = reinterpret_cast<uint32_t*>(w.dplus + (uint64_t)n + 1);
+    w.stmp = w.dplus + (uint64_t)n + 1 + w.tiles;
     return w;
 }
 
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz) {
     uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
-    return 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 +
+    return 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 + tiles +
            std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1));
 }
 
@@ -400,7 +418,9 @@ void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const ui
     k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, err);
     fill_u64(w.dplus, (uint64_t)n + 1, 0, st);
     if (w.tiles > 0) {
-        k_orient_keep<<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.bits, w.dplus,
+        fill_u32(w.tbnd, 2 * w.tiles, 0, st);
+        k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, w.tbnd);
+        k_orient_keep<<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.tbnd, w.bits, w.dplus,
                                                                w.tile_cnt, err);
         scan_u64(w.tile_cnt, w.tile_off, w.tiles, w.stmp, st);
     } else {
