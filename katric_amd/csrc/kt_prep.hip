@@ -179,41 +179,75 @@ __global__ void __launch_bounds__(OR_THREADS)
     const uint64_t nrows = (uint64_t)hi - lo + 1;
     const bool lds_rows = nrows <= OR_TILE;
     uint32_t flags = 0, cnt = 0;
-#pragma unroll 4
+    // The 64 slots of a wave are consecutive: the next slot's head and row
+    // come from the next lane (the last lane loads them), and kept slots are
+    // counted per run of equal rows with one LDS atomic from the run's last lane.
+    const unsigned lane = lane_id();
+    // all loads first (16 heads, then 32 degree gathers), so that a thread
+    // has every item's memory traffic in flight at once
+    uint32_t hu[OR_ITEMS], du[OR_ITEMS], dv[OR_ITEMS];
+#pragma unroll
+    for (int k = 0; k < OR_ITEMS; ++k) {
+        const uint64_t sl = tile0 + k * OR_THREADS + tid;
+        hu[k] = sl < tend ? heads[sl] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < OR_ITEMS; ++k) {
+        const unsigned i = k * OR_THREADS + tid;
+        const bool in = tile0 + i < tend;
+        du[k] = (in && hu[k] < n) ? deg[hu[k]] : 0u;
+        dv[k] = in ? deg[lo + marks[i]] : 0u;
+    }
+#pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
         const unsigned i = k * OR_THREADS + tid;
         const uint64_t sl = tile0 + i;
+        const bool in = sl < tend;
         bool keep = false;
-        if (sl < tend) {
-            const uint32_t rr = marks[i];
-            const uint32_t v = lo + rr;
-            const uint32_t u = heads[sl];
+        const uint32_t rr = in ? marks[i] : 0xffffffffu;
+        const uint32_t v = lo + rr;
+        const uint32_t u = hu[k];
+        if (in) {
             if (u >= n) {
                 flags |= KT_ERR_HEAD_RANGE;
             } else {
                 if (u == v) flags |= KT_ERR_SELF_LOOP;
-                keep = keep_edge(v, deg[v], u, deg[u]);
-            }
-            // next slot in the same row?
-            const bool same = (i + 1 < OR_TILE) ? (sl + 1 < tend && marks[i + 1] == rr)
-                                                 : (sl + 1 < nnz && sl + 1 < off[v + 1]);
-            if (same) {
-                const uint32_t nx = heads[sl + 1];
-                if (nx == u)
-                    flags |= KT_ERR_DUPLICATE;
-                else if (nx < u)
-                    flags |= KT_FLAG_UNSORTED;
-            }
-            if (keep) {
-                ++cnt;
-                if (lds_rows)
-                    atomicAdd(&rowcnt[rr], 1u);
-                else
-                    atomic_add_u64(&dplus[v], 1);
+                keep = keep_edge(v, dv[k], u, du[k]);
             }
         }
+        // next slot in the same row? (shuffles outside any branch: every lane active)
+        uint32_t nx = __shfl_down(u, 1, 64);
+        const uint32_t rr_next = __shfl_down(rr, 1, 64);
+        bool same = rr_next == rr;
+        if (lane == KT_WAVE - 1) {
+            same = in && ((i + 1 < OR_TILE) ? (sl + 1 < tend && marks[i + 1] == rr)
+                                            : (sl + 1 < nnz && sl + 1 < off[v + 1]));
+            if (same) nx = heads[sl + 1];
+        }
+        if (in && same) {
+            if (nx == u)
+                flags |= KT_ERR_DUPLICATE;
+            else if (nx < u)
+                flags |= KT_FLAG_UNSORTED;
+        }
         const uint64_t bits = __ballot(keep);
-        if (lane_id() == 0 && tile0 + k * OR_THREADS + (tid & ~63u) < tend)
+        cnt += keep ? 1u : 0u;
+        // run ends: the last lane, the last slot, or the next lane starts another row
+        const bool end = in && (lane == KT_WAVE - 1 || sl + 1 >= tend || rr_next != rr);
+        const uint64_t ends = __ballot(end);
+        if (end) {
+            const uint64_t below = ends & lanemask_lt();
+            const uint64_t upto = (lane == KT_WAVE - 1) ? ~0ull : ((2ull << lane) - 1);
+            const uint64_t run = upto & (below ? ~((2ull << (63 - __clzll((long long)below))) - 1) : ~0ull);
+            const uint32_t c = (uint32_t)__popcll(bits & run);
+            if (c) {
+                if (lds_rows)
+                    atomicAdd(&rowcnt[rr], c);
+                else
+                    atomic_add_u64(&dplus[v], c);
+            }
+        }
+        if (lane == 0 && tile0 + k * OR_THREADS + (tid & ~63u) < tend)
             keep_bits[(tile0 + k * OR_THREADS + (tid & ~63u)) / 64] = bits;
     }
     if (flags) atomicOr(err, flags);
