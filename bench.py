@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--ranks", type=int, default=16, help="shard mode: number of contiguous vertex ranges")
     ap.add_argument("--rank", type=int, default=0, help="shard mode: the rank counted")
     ap.add_argument("--ditric", action="store_true", help="shard mode: DiTriC (patric) attribution instead of CeTriC")
+    ap.add_argument("--all-ranks", action="store_true",
+                    help="shard mode: also count every rank's shard and check the partition identities")
     ap.add_argument("--graphs", type=int, default=8192, help="batch mode: number of graphs")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -205,7 +207,36 @@ def run_shard(args):
             "sample": f"full rank {args.rank} simulation (whole-graph degree orientation, ghost lists, message "
                       f"lists, local + contracted phase, {thr} OpenMP threads), {cpu_s:.2f} s",
         }
+    if args.all_ranks:
+        result["config"]["all_ranks"] = shard_sweep(args, ctx, g, algo, ds, d_pv)
     print(json.dumps(result), flush=True)
+
+
+def shard_sweep(args, ctx, g, algo, ds, d_pv):
+    """Counts every rank's shard on the GPU and checks the size-independent
+    identities of the partition against the one-graph count of the whole
+    graph: sum over ranks of local + type-3 triangles = total, and the
+    per-rank per-vertex counts summed over ranks = the per-vertex counts."""
+    from katric_amd import shard as S
+
+    for b in list(ds.bufs.values()) + [d_pv]:  # the rank-0 device inputs of the timed run
+        b.free()
+    total, pv = ctx.count(g)
+    acc = np.zeros(g.n, np.uint64)
+    per_rank, t_all = [], 0
+    for r in range(args.ranks):
+        t0 = time.time()
+        s = S.make_shard(g, args.ranks, r, algo)
+        loc, t3, pvr = S.count_shard(ctx, s)
+        np.add.at(acc, s.vertex_ids(), pvr)
+        per_rank.append([loc, t3])
+        t_all += loc + t3
+        log(f"[bench] rank {r}: local {loc} type-3 {t3} ({time.time() - t0:.1f}s)")
+    assert t_all == total, f"sum over ranks {t_all} != total {total}"
+    assert np.array_equal(acc, pv), "per-vertex counts summed over ranks differ from the one-graph counts"
+    return {"per_rank_local_type3": per_rank, "total": total,
+            "checked": "sum over ranks of local + type-3 = one-graph total; per-vertex counts summed over ranks "
+                       "= one-graph per-vertex counts"}
 
 
 def run_copy(args):
