@@ -208,35 +208,49 @@ def run_shard(args):
                       f"lists, local + contracted phase, {thr} OpenMP threads), {cpu_s:.2f} s",
         }
     if args.all_ranks:
-        result["config"]["all_ranks"] = shard_sweep(args, ctx, g, algo, ds, d_pv)
+        result["config"]["all_ranks"] = shard_sweep(args, ctx, g, algo, ds, d_pv, res)
     print(json.dumps(result), flush=True)
 
 
-def shard_sweep(args, ctx, g, algo, ds, d_pv):
-    """Counts every rank's shard on the GPU and checks the size-independent
-    identities of the partition against the one-graph count of the whole
-    graph: sum over ranks of local + type-3 triangles = total, and the
-    per-rank per-vertex counts summed over ranks = the per-vertex counts."""
+def shard_sweep(args, ctx, g, algo, ds, d_pv, rank_result):
+    """Counts every rank on the GPU from one resident copy of the whole graph
+    (kt_count_shard_graph builds each rank's inputs on the device) and checks
+    the size-independent identities of the partition against the one-graph
+    count: sum over ranks of local + type-3 triangles = total, and the
+    per-vertex counts accumulated over all ranks = the per-vertex counts.  The
+    timed rank's device-produced counts must equal its host-produced ones."""
     from katric_amd import shard as S
 
-    for b in list(ds.bufs.values()) + [d_pv]:  # the rank-0 device inputs of the timed run
+    for b in list(ds.bufs.values()) + [d_pv]:  # the timed rank's device inputs
         b.free()
     total, pv = ctx.count(g)
-    acc = np.zeros(g.n, np.uint64)
-    per_rank, t_all = [], 0
+    dg = ctx.upload(g)
+    d_acc = ctx.alloc(g.n * 8)
+    d_acc.upload(np.zeros(g.n, np.uint64))
+    per_rank, rank_ms, t_all = [], [], 0
+    ctx.synchronize()
+    t_sweep = time.perf_counter()
     for r in range(args.ranks):
-        t0 = time.time()
-        s = S.make_shard(g, args.ranks, r, algo)
-        loc, t3, pvr = S.count_shard(ctx, s)
-        np.add.at(acc, s.vertex_ids(), pvr)
+        t0 = time.perf_counter()
+        loc, t3, _ = S.count_shard_graph(ctx, dg, args.ranks, r, algo, per_vertex_out=d_acc, accumulate=True)
+        tm = ctx.timing()
         per_rank.append([loc, t3])
+        rank_ms.append(round(tm["total_ms"], 3))
         t_all += loc + t3
-        log(f"[bench] rank {r}: local {loc} type-3 {t3} ({time.time() - t0:.1f}s)")
+        log(f"[bench] rank {r}: local {loc} type-3 {t3} device {tm['total_ms']:.1f} ms "
+            f"(producer + prep {tm['prep_ms']:.1f}, count {tm['count_ms']:.1f}), wall {(time.perf_counter() - t0) * 1e3:.0f} ms")
+        if r == args.rank:
+            assert (loc, t3) == rank_result, f"device producer {loc, t3} != host producer {rank_result}"
+    sweep_ms = (time.perf_counter() - t_sweep) * 1e3
+    acc = d_acc.download(np.uint64, g.n)
     assert t_all == total, f"sum over ranks {t_all} != total {total}"
     assert np.array_equal(acc, pv), "per-vertex counts summed over ranks differ from the one-graph counts"
-    return {"per_rank_local_type3": per_rank, "total": total,
-            "checked": "sum over ranks of local + type-3 = one-graph total; per-vertex counts summed over ranks "
-                       "= one-graph per-vertex counts"}
+    d_acc.free()
+    dg.free()
+    return {"per_rank_local_type3": per_rank, "per_rank_device_ms": rank_ms, "sweep_wall_ms": sweep_ms,
+            "total": total, "producer": "device (kt_count_shard_graph from the resident full graph)",
+            "checked": "sum over ranks of local + type-3 = one-graph total; per-vertex counts accumulated over ranks "
+                       "= one-graph per-vertex counts; the timed rank equals its host-produced shard"}
 
 
 def run_copy(args):

@@ -70,6 +70,28 @@ typedef struct {
  * cetric.h:293/979). */
 int kt_count_shard(kt_ctx* ctx, const kt_shard_in* in, kt_shard_out* out);
 
+/* One rank's counts from a full symmetric CSR (n vertices, offsets[n+1],
+ * heads[offsets[n]], ascending or not; device-resident with KT_DEVICE_INPUT):
+ * the rank's kt_shard_in -- local slice, ghost ids and global degrees, the
+ * senders of the distributed phase and their message lists -- is built on the
+ * device from the resident graph (what the reference's MPI exchange delivers:
+ * the 1D partitioned read, DegreeCommunicator,
+ * datastructures/distributed/graph_communicator.h:48-266, and the message
+ * queue of the contracted / distributed phase,
+ * counters/cetric_edge_iterator.h:1328-1379, 1632-1661; ghosts as in
+ * find_ghosts, distributed_graph.h:1287-1298), then counted exactly like
+ * kt_count_shard.  out->per_vertex (nullable) has n entries indexed by global
+ * id: the triangles emitted on this rank containing each vertex (zero for
+ * vertices the rank does not see); with KT_SHARD_ACCUMULATE they are added to
+ * its contents instead, so a sweep over all ranks sums to kt_count's
+ * per-vertex counts.  range_bounds is a host array as in kt_shard_in.  No
+ * single reference counterpart: it fuses the exchange of the inputs of one
+ * rank with run_cetric_new / run_patric on that rank. */
+#define KT_SHARD_ACCUMULATE 0x10u
+int kt_count_shard_graph(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+                         uint32_t algorithm, uint32_t num_ranks, const uint64_t* range_bounds, uint32_t rank,
+                         kt_shard_out* out);
+
 /* Host-side producer of kt_shard_in arrays for one rank from a full
  * symmetric CSR (it computes what MPI delivers to that rank in the reference:
  * the local CSR slice, the ghost degrees and the received messages). */

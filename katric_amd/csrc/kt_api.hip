@@ -579,6 +579,7 @@ extern "C" int kt_synchronize(kt_ctx* c) {
 void kt_ctx::destroy_extensions() {
     kt_serve_release(this);
     kt_batch_release(this);
+    kt_shard_graph_release(this);
     for (auto& kv : buffers) (void)hipFree(kv.first);
     buffers.clear();
 }

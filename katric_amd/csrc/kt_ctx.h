@@ -7,6 +7,7 @@
 #include <unordered_map>
 
 #include "../../include/katric.h"
+#include "../../include/katric_shard.h"
 #include "kt_internal.h"
 
 enum {
@@ -52,6 +53,10 @@ struct kt_ctx {
     kt::ServeState* serve = nullptr;
     kt::BatchState* batch = nullptr;
     std::unordered_map<void*, uint64_t> buffers;  // kt_buffer_alloc allocations
+    // grow-only device buffers of the device-side shard producer: [0] sized
+    // by the graph and the rank's slice, [1] by the received message volume
+    void* prod[2] = {nullptr, nullptr};
+    uint64_t prod_bytes[2] = {0, 0};
 
     int fail(int status, const char* fmt, ...);
     int hip_check(hipError_t e, const char* what);
@@ -63,9 +68,12 @@ struct kt_ctx {
 
 void kt_batch_release(kt_ctx* c);
 void kt_serve_release(kt_ctx* c);
+void kt_shard_graph_release(kt_ctx* c);
 
 namespace kt {
 uint64_t* tc_stats_ptr(uint64_t* ws);
+// kt_count_shard without taking the context lock (kt_shard.hip)
+int count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out);
 // KATRIC_TRACE=1: one stderr line per host-side pipeline step
 bool trace_enabled();
 void trace(const char* fmt, ...);

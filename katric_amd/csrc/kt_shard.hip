@@ -333,11 +333,9 @@ uint64_t shard_layout(void* base, const kt_shard_in* in, uint64_t nnz, uint64_t 
 
 }  // namespace
 
-extern "C" int kt_count_shard(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) {
-    if (!c) return KT_EINVAL;
-    std::lock_guard<std::mutex> lk(c->mu);
-    int rc = c->begin_call(true);
-    if (rc) return rc;
+// The body of kt_count_shard; the caller holds c->mu and has begun the call.
+int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) {
+    int rc = KT_OK;
     if (!in || !out || !in->range_bounds || !in->offsets || in->num_ranks == 0 || in->rank >= in->num_ranks ||
         in->algorithm > KT_SHARD_DITRIC)
         return c->fail(KT_EINVAL, "bad shard arguments");
@@ -537,4 +535,12 @@ extern "C" int kt_count_shard(kt_ctx* c, const kt_shard_in* in, kt_shard_out* ou
     c->timing.d2h_ms = c->elapsed(EV_COUNT, EV_END);
     c->timing.total_ms = c->elapsed(EV_START, EV_END);
     return KT_OK;
+}
+
+extern "C" int kt_count_shard(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) {
+    if (!c) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(true);
+    if (rc) return rc;
+    return count_shard_locked(c, in, out);
 }

@@ -1,0 +1,526 @@
+// Device-side producer of one rank's shard inputs (kt_count_shard_graph).
+//
+// From a full symmetric CSR resident in HBM the rank's kt_shard_in is built
+// on the device -- what the reference's MPI exchange delivers to the rank --
+// and then counted by the kt_count_shard pipeline:
+//   1. global degrees deg[v] of every vertex (offsets validated);
+//   2. one pass over the rank's slots [off[from], off[to]): a bit per
+//      non-owned head (its ghosts, find_ghosts, distributed_graph.h:1287-1298)
+//      and a bit per ghost a with (deg a, a) < (deg x, x) for an owned
+//      neighbour x (a has an out-edge into the rank, so a's owner sends
+//      [a, list] to it, cetric_edge_iterator.h:1328-1379, 1632-1661);
+//   3. both bitmaps compacted in id order (popcount per word, scan, write):
+//      ghost ids ascending with their global degrees (DegreeCommunicator,
+//      graph_communicator.h:48-266), senders ascending;
+//   4. per sender a, a group of 16 lanes walks a's full list and keeps the x
+//      with (deg a, a) < (deg x, x); cetric-x also drops the x owned by a's
+//      owner (N+cut(a)), patric keeps all of N+(a).  Kept ids are counted, the
+//      counts scanned and the ids written in list order; lists that are not
+//      ascending are put in order by one key sort over (message, id).
+// The rank's slice is not copied: its offsets are rebased into a new array and
+// its heads are the full graph's heads from off[from] on.  The senders are
+// found from the rank's own slots; for a symmetric CSR that is the same set
+// as walking every ghost's list.
+#include <algorithm>
+
+#include "../../include/katric_shard.h"
+#include "kt_ctx.h"
+#include "kt_device.h"
+
+namespace kt {
+
+constexpr int PD_THREADS = 256;
+constexpr int PD_ITEMS = 16;
+constexpr int PD_TILE = PD_THREADS * PD_ITEMS;
+constexpr int PD_GROUP = 16;  // lanes per sender list walk
+
+__device__ __forceinline__ bool before(uint32_t da, uint32_t a, uint32_t db, uint32_t b) {
+    return da < db || (da == db && a < b);
+}
+
+__device__ __forceinline__ void set_bit(uint32_t* bits, uint32_t u) {
+    const uint32_t w = u >> 5, b = 1u << (u & 31);
+    if (!(bits[w] & b)) atomicOr(&bits[w], b);  // a stale read only costs a redundant atomic
+}
+
+__global__ void k_pd_degree(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, uint32_t* __restrict__ deg,
+                            uint32_t* __restrict__ err) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        if (v == 0 && (off[0] != 0 || off[n] != nnz)) atomicOr(err, KT_ERR_OFFSETS);
+        const uint64_t a = off[v], b = off[v + 1];
+        if (b < a || b - a > 0xffffffffull || b > nnz) {
+            atomicOr(err, KT_ERR_OFFSETS);
+            deg[v] = 0;
+        } else {
+            deg[v] = (uint32_t)(b - a);
+        }
+    }
+}
+
+// Slots [base, end) of the rows [from, to); soff = off + from.
+__global__ void __launch_bounds__(PD_THREADS)
+    k_pd_mark(const uint64_t* __restrict__ soff, uint32_t nl, uint32_t from, uint32_t n, uint64_t base, uint64_t end,
+              const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg, uint32_t* __restrict__ gbits,
+              uint32_t* __restrict__ sbits, uint32_t* __restrict__ err) {
+    __shared__ uint32_t marks[PD_TILE];
+    __shared__ uint32_t wmax[PD_THREADS / KT_WAVE];
+    __shared__ uint32_t bnd[2];
+    const uint64_t tile0 = base + (uint64_t)blockIdx.x * PD_TILE;
+    const uint64_t tend = min(tile0 + PD_TILE, end);
+    uint32_t lo, hi;
+    tile_rows<PD_THREADS, PD_ITEMS>(soff, nl, tile0, tend, marks, bnd, wmax, lo, hi);
+    const uint32_t to = from + nl;
+    uint32_t flags = 0;
+    for (int k = 0; k < PD_ITEMS; ++k) {
+        const unsigned i = k * PD_THREADS + threadIdx.x;
+        const uint64_t sl = tile0 + i;
+        if (sl >= tend) continue;
+        const uint32_t x = from + lo + marks[i];
+        const uint32_t u = heads[sl];
+        if (u >= n) {
+            flags |= KT_ERR_HEAD_RANGE;
+        } else if (u < from || u >= to) {
+            set_bit(gbits, u);
+            if (before(deg[u], u, deg[x], x)) set_bit(sbits, u);
+        }
+    }
+    if (flags) atomicOr(err, flags);
+}
+
+__global__ void k_pd_popc(uint64_t words, const uint32_t* __restrict__ gbits, const uint32_t* __restrict__ sbits,
+                          uint64_t* __restrict__ gcnt, uint64_t* __restrict__ scnt) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+        gcnt[w] = __popc(gbits[w]);
+        scnt[w] = __popc(sbits[w]);
+    }
+}
+
+__global__ void k_pd_compact(uint64_t words, const uint32_t* __restrict__ gbits, const uint64_t* __restrict__ goff,
+                             const uint32_t* __restrict__ sbits, const uint64_t* __restrict__ soff,
+                             const uint64_t* __restrict__ off, uint32_t* __restrict__ ghost_ids,
+                             uint64_t* __restrict__ ghost_deg, uint32_t* __restrict__ msg_src) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+        uint32_t bits = gbits[w];
+        uint64_t p = goff[w];
+        while (bits) {
+            const uint32_t id = (uint32_t)(w * 32) + (__ffs(bits) - 1);
+            bits &= bits - 1;
+            ghost_ids[p] = id;
+            ghost_deg[p] = off[id + 1] - off[id];
+            ++p;
+        }
+        bits = sbits[w];
+        p = soff[w];
+        while (bits) {
+            msg_src[p++] = (uint32_t)(w * 32) + (__ffs(bits) - 1);
+            bits &= bits - 1;
+        }
+    }
+}
+
+__global__ void k_pd_rebase(uint32_t nl, const uint64_t* __restrict__ soff, uint64_t* __restrict__ loff) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= nl; i += stride) loff[i] = soff[i] - soff[0];
+}
+
+// [lo, hi): the id range of the owner of a, the first rank r with a < bd[r+1]
+// (as the reference's rank lookup over range ends, util/helpers.h:83-101).
+__device__ __forceinline__ void owner_range(const uint64_t* __restrict__ bd, uint32_t P, uint32_t a, uint64_t& lo,
+                                            uint64_t& hi) {
+    uint32_t l = 0, h = P;
+    while (l < h) {
+        const uint32_t mid = (l + h) >> 1;
+        if (a < bd[mid + 1])
+            h = mid;
+        else
+            l = mid + 1;
+    }
+    const uint32_t o = min(l, P - 1);
+    lo = bd[o];
+    hi = bd[o + 1];
+}
+
+// One 16-lane group walks sender a's full list: `kept` receives the number of
+// ids its message keeps; WRITE also stores them from out[pos] on, in list
+// order.  Without WRITE, a list that is not strictly ascending sets
+// KT_FLAG_UNSORTED in `flags`.
+template <bool WRITE>
+__device__ __forceinline__ void walk_sender(uint32_t a, const uint64_t* __restrict__ off,
+                                            const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg,
+                                            uint32_t n, const uint64_t* __restrict__ bd, uint32_t P, bool cetric,
+                                            uint64_t pos, uint32_t* __restrict__ out, uint64_t& kept,
+                                            uint32_t& flags) {
+    const unsigned lane = lane_id();
+    const unsigned gl = lane & (PD_GROUP - 1);
+    const uint64_t gmask = 0xffffull << (lane & ~(PD_GROUP - 1));
+    const uint64_t b = off[a], e = off[a + 1];
+    const uint32_t da = deg[a];
+    uint64_t lo = 0, hi = 0;  // patric: nothing excluded
+    if (cetric) owner_range(bd, P, a, lo, hi);
+    kept = 0;
+    for (uint64_t p0 = b; p0 < e; p0 += PD_GROUP) {  // uniform over the group
+        const uint64_t p = p0 + gl;
+        bool keep = false;
+        uint32_t x = 0;
+        if (p < e) {
+            x = heads[p];
+            if (x >= n) {
+                flags |= KT_ERR_HEAD_RANGE;
+            } else {
+                keep = before(da, a, deg[x], x) && !(x >= lo && x < hi);
+                if (!WRITE && p + 1 < e && heads[p + 1] <= x) flags |= KT_FLAG_UNSORTED;
+            }
+        }
+        const uint64_t m = __ballot(keep) & gmask;
+        if (WRITE && keep) out[pos + kept + __popcll(m & lanemask_lt())] = x;
+        kept += __popcll(m);
+    }
+}
+
+__global__ void k_pd_msg_len(uint32_t nmsg, const uint32_t* __restrict__ src, const uint64_t* __restrict__ off,
+                             const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg, uint32_t n,
+                             const uint64_t* __restrict__ bd, uint32_t P, int cetric, uint64_t* __restrict__ len,
+                             uint32_t* __restrict__ err) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / PD_GROUP;
+    uint32_t flags = 0;
+    for (uint64_t i = t / PD_GROUP; i < nmsg; i += groups) {
+        uint64_t kept;
+        walk_sender<false>(src[i], off, heads, deg, n, bd, P, cetric != 0, 0, nullptr, kept, flags);
+        if ((t & (PD_GROUP - 1)) == 0) len[i] = kept;
+    }
+    if (flags) atomicOr(err, flags);
+}
+
+__global__ void k_pd_msg_fill(uint32_t nmsg, const uint32_t* __restrict__ src, const uint64_t* __restrict__ off,
+                              const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg, uint32_t n,
+                              const uint64_t* __restrict__ bd, uint32_t P, int cetric,
+                              const uint64_t* __restrict__ moff, uint32_t* __restrict__ mheads) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / PD_GROUP;
+    uint32_t flags = 0;
+    for (uint64_t i = t / PD_GROUP; i < nmsg; i += groups) {
+        uint64_t kept;
+        walk_sender<true>(src[i], off, heads, deg, n, bd, P, cetric != 0, moff[i], mheads, kept, flags);
+    }
+}
+
+// keys[p] = (message << 32) | id over every message entry
+__global__ void k_pd_keys(uint32_t nmsg, const uint64_t* __restrict__ moff, const uint32_t* __restrict__ mheads,
+                          uint64_t* __restrict__ keys) {
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KT_WAVE;
+    const uint32_t nw = gridDim.x * blockDim.x / KT_WAVE;
+    for (uint32_t i = wid; i < nmsg; i += nw)
+        for (uint64_t p = moff[i] + lane_id(); p < moff[i + 1]; p += KT_WAVE) keys[p] = ((uint64_t)i << 32) | mheads[p];
+}
+
+__global__ void k_pd_keys_low(uint64_t len, const uint64_t* __restrict__ keys, uint32_t* __restrict__ mheads) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < len; p += stride)
+        mheads[p] = (uint32_t)keys[p];
+}
+
+// Per-vertex counts of the rank's combined index space (owned, then ghosts)
+// added to the global-id array.
+__global__ void k_pd_scatter(uint32_t nl, uint32_t ng, uint32_t from, const uint32_t* __restrict__ ghost_ids,
+                             const uint64_t* __restrict__ pvx, uint64_t* __restrict__ out) {
+    const uint64_t X = (uint64_t)nl + ng;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < X; c += stride) {
+        const uint32_t g = c < nl ? from + (uint32_t)c : ghost_ids[c - nl];
+        out[g] += pvx[c];
+    }
+}
+
+}  // namespace kt
+
+using namespace kt;
+
+namespace {
+
+unsigned grid_for(uint64_t items, unsigned per_block = 256, unsigned cap = 16384) {
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((items + per_block - 1) / per_block, cap));
+}
+
+int bits_of(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b)) ++b;
+    return b;
+}
+
+// Grows producer buffer i to at least `bytes` (its contents are dropped).
+int prod_reserve(kt_ctx* c, int i, uint64_t bytes) {
+    bytes = (std::max<uint64_t>(bytes, 256) + 255) & ~255ull;
+    if (bytes <= c->prod_bytes[i]) return KT_OK;
+    uint64_t want = std::max<uint64_t>(bytes, c->prod_bytes[i] + c->prod_bytes[i] / 4);
+    const uint64_t others = c->arena_bytes + c->pinned_other - c->prod_bytes[i];
+    if (c->budget && others + want > c->budget) want = bytes;
+    if (c->budget && others + want > c->budget)
+        return c->fail(KT_ENOMEM, "shard producer needs %llu bytes beyond the HBM budget of %llu bytes",
+                       (unsigned long long)want, (unsigned long long)c->budget);
+    if (c->prod[i]) {
+        int rc = c->hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        if (rc) return rc;
+        (void)hipFree(c->prod[i]);
+        c->pinned_other -= c->prod_bytes[i];
+        c->prod[i] = nullptr;
+        c->prod_bytes[i] = 0;
+    }
+    hipError_t e = hipMalloc(&c->prod[i], want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c->prod[i] = nullptr;
+        return c->fail(KT_ENOMEM, "hipMalloc(%llu) for the shard producer failed", (unsigned long long)want);
+    }
+    c->prod_bytes[i] = want;
+    c->pinned_other += want;
+    return KT_OK;
+}
+
+struct ProdBufs {
+    uint64_t* small;
+    uint64_t* bd;
+    uint32_t* deg;
+    uint32_t* bits;  // ghost bits, then sender bits
+    uint64_t* gcnt;
+    uint64_t* scnt;
+    uint64_t* stmp;
+    uint64_t* loff;
+    uint32_t* ghost_ids;
+    uint64_t* ghost_deg;
+    uint32_t* msg_src;
+    uint64_t* moff;
+    uint64_t* pvx;
+    uint64_t* gpv;
+    uint64_t* hoff;
+    uint32_t* hheads;
+};
+
+uint64_t prod_layout(void* base, uint32_t n, uint64_t nnz, uint32_t P, uint32_t nl, uint64_t ng_max, bool want_pv,
+                     bool host_pv, bool host_in, ProdBufs& B) {
+    Carver cv{base, 0};
+    const uint64_t W = ((uint64_t)n + 31) / 32;
+    B.small = static_cast<uint64_t*>(cv.take(64 * 8));
+    B.bd = static_cast<uint64_t*>(cv.take(((uint64_t)P + 1) * 8));
+    B.deg = static_cast<uint32_t*>(cv.take((uint64_t)n * 4));
+    B.bits = static_cast<uint32_t*>(cv.take(2 * W * 4));
+    B.gcnt = static_cast<uint64_t*>(cv.take((W + 1) * 8));
+    B.scnt = static_cast<uint64_t*>(cv.take((W + 1) * 8));
+    B.stmp = static_cast<uint64_t*>(cv.take(scan_tmp_words(std::max<uint64_t>(W, ng_max) + 1) * 8));
+    B.loff = static_cast<uint64_t*>(cv.take(((uint64_t)nl + 1) * 8));
+    B.ghost_ids = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(ng_max, 1) * 4));
+    B.ghost_deg = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(ng_max, 1) * 8));
+    B.msg_src = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(ng_max, 1) * 4));
+    B.moff = static_cast<uint64_t*>(cv.take((ng_max + 1) * 8));
+    B.pvx = want_pv ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(nl + ng_max, 1) * 8)) : nullptr;
+    B.gpv = host_pv ? static_cast<uint64_t*>(cv.take((uint64_t)n * 8)) : nullptr;
+    B.hoff = host_in ? static_cast<uint64_t*>(cv.take(((uint64_t)n + 1) * 8)) : nullptr;
+    B.hheads = host_in ? static_cast<uint32_t*>(cv.take(std::max<uint64_t>(nnz, 1) * 4)) : nullptr;
+    return cv.used;
+}
+
+int read_flags(kt_ctx* c, const uint64_t* small, int words, const char* what) {
+    int rc = c->hip_check(hipMemcpyAsync(c->host_small, small, words * 8, hipMemcpyDeviceToHost, c->stream), what);
+    if (rc) return rc;
+    return c->hip_check(hipStreamSynchronize(c->stream), what);
+}
+
+int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+                             uint32_t algorithm, uint32_t P, const uint64_t* bd, uint32_t rank, kt_shard_out* out) {
+    if (!out || !offsets || !bd || P == 0 || rank >= P || algorithm > KT_SHARD_DITRIC)
+        return c->fail(KT_EINVAL, "bad shard arguments");
+    if (bd[0] != 0 || bd[P] != n) return c->fail(KT_EINVAL, "range_bounds must cover [0, n)");
+    for (uint32_t r = 0; r < P; ++r)
+        if (bd[r + 1] < bd[r]) return c->fail(KT_EINVAL, "range bounds not ascending");
+    out->local_triangles = out->type3_triangles = 0;
+    if (n == 0) return KT_OK;
+    const bool host_in = !(flags & KT_DEVICE_INPUT);
+    const bool want_pv = out->per_vertex != nullptr;
+    const bool pv_dev = want_pv && (flags & KT_DEVICE_OUTPUT);
+    const bool acc = (flags & KT_SHARD_ACCUMULATE) != 0;
+    const bool cetric = algorithm == KT_SHARD_CETRIC;
+    const uint32_t from = (uint32_t)bd[rank], to = (uint32_t)bd[rank + 1], nl = to - from;
+    hipStream_t st = c->stream;
+    int rc;
+    // sizes of the graph and of the rank's slice
+    uint64_t nnz, base, end;
+    if (host_in) {
+        nnz = offsets[n];
+        base = offsets[from];
+        end = offsets[to];
+    } else {
+        const uint64_t* src[3] = {offsets + n, offsets + from, offsets + to};
+        for (int i = 0; i < 3; ++i) {
+            rc = c->hip_check(hipMemcpyAsync(c->host_small + i, src[i], 8, hipMemcpyDeviceToHost, st), "D2H sizes");
+            if (rc) return rc;
+        }
+        rc = c->hip_check(hipStreamSynchronize(st), "D2H sizes");
+        if (rc) return rc;
+        nnz = c->host_small[0];
+        base = c->host_small[1];
+        end = c->host_small[2];
+    }
+    if (base > end || end > nnz || (nnz >> 40)) return c->fail(KT_EINVAL, "malformed CSR: bad offsets");
+    if (nnz && !heads) return c->fail(KT_EINVAL, "null heads");
+    const uint64_t nnz_l = end - base;
+    const uint64_t ng_max = std::min<uint64_t>((uint64_t)n - nl, nnz_l);
+    ProdBufs B;
+    const uint64_t need = prod_layout(nullptr, n, nnz, P, nl, ng_max, want_pv, want_pv && !pv_dev, host_in, B);
+    rc = prod_reserve(c, 0, need);
+    if (rc) return rc;
+    prod_layout(c->prod[0], n, nnz, P, nl, ng_max, want_pv, want_pv && !pv_dev, host_in, B);
+    (void)hipEventRecord(c->ev[EV_X0], st);
+    const uint64_t* off = offsets;
+    const uint32_t* hd = heads;
+    if (host_in) {
+        rc = c->hip_check(hipMemcpyAsync(B.hoff, offsets, ((uint64_t)n + 1) * 8, hipMemcpyHostToDevice, st), "H2D");
+        if (!rc && nnz) rc = c->hip_check(hipMemcpyAsync(B.hheads, heads, nnz * 4, hipMemcpyHostToDevice, st), "H2D");
+        if (rc) return rc;
+        off = B.hoff;
+        hd = B.hheads;
+    }
+    rc = c->hip_check(hipMemcpyAsync(B.bd, bd, ((uint64_t)P + 1) * 8, hipMemcpyHostToDevice, st), "H2D bounds");
+    if (rc) return rc;
+    uint32_t* err = reinterpret_cast<uint32_t*>(B.small);
+    const uint64_t W = ((uint64_t)n + 31) / 32;
+    fill_u64(B.small, 64, 0, st);
+    fill_u32(B.bits, 2 * W, 0, st);
+    k_pd_degree<<<grid_for(n), 256, 0, st>>>(n, nnz, off, B.deg, err);
+    rc = read_flags(c, B.small, 1, "degrees");
+    if (rc) return rc;
+    if (c->host_small[0] & KT_ERR_MASK) return c->fail(KT_EINVAL, "malformed CSR: bad offsets");
+    // ghosts and senders
+    const uint64_t tiles = (nnz_l + PD_TILE - 1) / PD_TILE;
+    if (nl && tiles)
+        k_pd_mark<<<(unsigned)tiles, PD_THREADS, 0, st>>>(off + from, nl, from, n, base, end, hd, B.deg, B.bits,
+                                                          B.bits + W, err);
+    k_pd_popc<<<grid_for(W), 256, 0, st>>>(W, B.bits, B.bits + W, B.gcnt, B.scnt);
+    scan_u64(B.gcnt, B.gcnt, W, B.stmp, st);
+    scan_u64(B.scnt, B.scnt, W, B.stmp, st);
+    k_pd_compact<<<grid_for(W), 256, 0, st>>>(W, B.bits, B.gcnt, B.bits + W, B.scnt, off, B.ghost_ids, B.ghost_deg,
+                                               B.msg_src);
+    k_pd_rebase<<<grid_for((uint64_t)nl + 1), 256, 0, st>>>(nl, off + from, B.loff);
+    rc = c->hip_check(hipMemcpyAsync(B.small + 1, B.gcnt + W, 8, hipMemcpyDeviceToDevice, st), "D2D counts");
+    if (!rc) rc = c->hip_check(hipMemcpyAsync(B.small + 2, B.scnt + W, 8, hipMemcpyDeviceToDevice, st), "D2D counts");
+    if (!rc) rc = read_flags(c, B.small, 3, "ghosts");
+    if (rc) return rc;
+    uint32_t fl = (uint32_t)c->host_small[0];
+    const uint64_t ng = c->host_small[1], nmsg = c->host_small[2];
+    if (fl & KT_ERR_MASK) return c->fail(KT_EINVAL, "malformed CSR:%s", (fl & KT_ERR_HEAD_RANGE) ? " head id >= n" : "");
+    if (ng > ng_max || nmsg > ng) return c->fail(KT_EINVAL, "shard producer: inconsistent ghost counts");
+    // message lists
+    uint64_t mtot = 0;
+    uint32_t* mheads = nullptr;
+    if (nmsg) {
+        const unsigned mgrid = grid_for(nmsg * PD_GROUP, 256, 8192);
+        k_pd_msg_len<<<mgrid, 256, 0, st>>>((uint32_t)nmsg, B.msg_src, off, hd, B.deg, n, B.bd, P, cetric ? 1 : 0,
+                                            B.moff, err);
+        scan_u64(B.moff, B.moff, nmsg, B.stmp, st);
+        rc = c->hip_check(hipMemcpyAsync(B.small + 1, B.moff + nmsg, 8, hipMemcpyDeviceToDevice, st), "D2D total");
+        if (!rc) rc = read_flags(c, B.small, 2, "message lengths");
+        if (rc) return rc;
+        fl = (uint32_t)c->host_small[0];
+        mtot = c->host_small[1];
+        if (fl & KT_ERR_MASK)
+            return c->fail(KT_EINVAL, "malformed CSR:%s", (fl & KT_ERR_HEAD_RANGE) ? " head id >= n" : "");
+        const bool unsorted = (fl & KT_FLAG_UNSORTED) != 0;
+        Carver cv{nullptr, 0};
+        cv.take(std::max<uint64_t>(mtot, 1) * 4);
+        if (unsorted) {
+            cv.take(mtot * 8);
+            cv.take(mtot * 8);
+            cv.take(radix_tmp_words(mtot) * 8);
+        }
+        rc = prod_reserve(c, 1, cv.used);
+        if (rc) return rc;
+        cv = Carver{c->prod[1], 0};
+        mheads = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(mtot, 1) * 4));
+        k_pd_msg_fill<<<mgrid, 256, 0, st>>>((uint32_t)nmsg, B.msg_src, off, hd, B.deg, n, B.bd, P, cetric ? 1 : 0,
+                                             B.moff, mheads);
+        if (unsorted && mtot) {
+            uint64_t* keys = static_cast<uint64_t*>(cv.take(mtot * 8));
+            uint64_t* alt = static_cast<uint64_t*>(cv.take(mtot * 8));
+            uint64_t* rtmp = static_cast<uint64_t*>(cv.take(radix_tmp_words(mtot) * 8));
+            k_pd_keys<<<grid_for(nmsg * KT_WAVE, 256, 8192), 256, 0, st>>>((uint32_t)nmsg, B.moff, mheads, keys);
+            radix_sort_u64(keys, alt, mtot, 32 + bits_of(nmsg), rtmp, st);
+            k_pd_keys_low<<<grid_for(mtot), 256, 0, st>>>(mtot, keys, mheads);
+        }
+    }
+    (void)hipEventRecord(c->ev[EV_X2], st);
+    // the rank's count on the produced inputs
+    kt_shard_in in{};
+    in.algorithm = algorithm;
+    in.rank = rank;
+    in.num_ranks = P;
+    in.flags = KT_DEVICE_INPUT;
+    in.range_bounds = bd;
+    in.offsets = B.loff;
+    in.heads = hd + base;
+    in.n_ghost = (uint32_t)ng;
+    in.ghost_ids = B.ghost_ids;
+    in.ghost_degree = B.ghost_deg;
+    in.n_msg = (uint32_t)nmsg;
+    in.msg_src = B.msg_src;
+    in.msg_offsets = B.moff;
+    in.msg_heads = mheads;
+    kt_shard_out o{0, 0, nullptr};
+    if (want_pv) {
+        in.flags |= KT_DEVICE_OUTPUT;
+        o.per_vertex = B.pvx;
+        fill_u64(B.pvx, (uint64_t)nl + ng, 0, st);  // stays zero if the rank sees nothing
+    }
+    rc = count_shard_locked(c, &in, &o);
+    if (rc) return rc;
+    if (want_pv) {
+        uint64_t* dst = pv_dev ? out->per_vertex : B.gpv;
+        if (!acc)
+            fill_u64(dst, n, 0, st);
+        else if (!pv_dev) {
+            rc = c->hip_check(hipMemcpyAsync(dst, out->per_vertex, (uint64_t)n * 8, hipMemcpyHostToDevice, st),
+                              "H2D per-vertex");
+            if (rc) return rc;
+        }
+        k_pd_scatter<<<grid_for((uint64_t)nl + ng), 256, 0, st>>>(nl, (uint32_t)ng, from, B.ghost_ids, B.pvx, dst);
+        if (!pv_dev) {
+            rc = c->hip_check(hipMemcpyAsync(out->per_vertex, dst, (uint64_t)n * 8, hipMemcpyDeviceToHost, st),
+                              "D2H per-vertex");
+            if (rc) return rc;
+        }
+    }
+    (void)hipEventRecord(c->ev[EV_X1], st);
+    rc = c->hip_check(hipStreamSynchronize(st), "shard graph");
+    if (rc) return rc;
+    out->local_triangles = o.local_triangles;
+    out->type3_triangles = o.type3_triangles;
+    // the producer counts as preprocessing (it stands for the exchange of
+    // the rank's inputs); total spans producer, count and per-vertex scatter
+    c->timing.prep_ms += c->elapsed(EV_X0, EV_X2);
+    c->timing.total_ms = c->elapsed(EV_X0, EV_X1);
+    return KT_OK;
+}
+
+}  // namespace
+
+void kt_shard_graph_release(kt_ctx* c) {
+    for (int i = 0; i < 2; ++i) {
+        if (c->prod[i]) {
+            (void)hipFree(c->prod[i]);
+            c->pinned_other -= c->prod_bytes[i];
+        }
+        c->prod[i] = nullptr;
+        c->prod_bytes[i] = 0;
+    }
+}
+
+extern "C" int kt_count_shard_graph(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads,
+                                    uint32_t flags, uint32_t algorithm, uint32_t num_ranks,
+                                    const uint64_t* range_bounds, uint32_t rank, kt_shard_out* out) {
+    if (!c) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(true);
+    if (rc) return rc;
+    return count_shard_graph_locked(c, n, offsets, heads, flags, algorithm, num_ranks, range_bounds, rank, out);
+}

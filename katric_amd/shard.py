@@ -166,3 +166,45 @@ def count_shard_device(ctx, d: DeviceShard, per_vertex_out=None):
     so = KtShardOut(0, 0, pv)
     ctx.check(lib().kt_count_shard(ctx.handle, C.byref(si), C.byref(so)), "kt_count_shard")
     return int(so.local_triangles), int(so.type3_triangles)
+
+
+ACCUMULATE = 0x10  # KT_SHARD_ACCUMULATE: per-vertex counts are added to the output
+
+_ffi.register("kt_count_shard_graph", C.c_int, [vp, u32, vp, vp, u32, u32, u32, vp, u32, C.POINTER(KtShardOut)])
+
+
+def count_shard_graph(ctx, g, num_ranks: int, rank: int, algorithm: int = CETRIC, bounds=None,
+                      per_vertex: bool = True, per_vertex_out=None, accumulate: bool = False):
+    """One rank's counts with its shard built on the device from the whole
+    graph `g` (a host Graph, or a DeviceGraph resident in HBM) --
+    kt_count_shard_graph.  Per-vertex counts are indexed by global id: into
+    the device buffer `per_vertex_out` (n entries) when given, else returned
+    as a host array when `per_vertex`; with `accumulate` they are added to the
+    output's contents (a host array passed as `per_vertex`, or the device
+    buffer).  Returns (local_triangles, type3_triangles, per_vertex or None)."""
+    from . import KT_DEVICE_INPUT, KT_DEVICE_OUTPUT, DeviceGraph
+
+    b = np.ascontiguousarray(bounds if bounds is not None else even_bounds(g.n, num_ranks), dtype=np.uint64)
+    flags = ACCUMULATE if accumulate else 0
+    if isinstance(g, DeviceGraph):
+        flags |= KT_DEVICE_INPUT
+        off, hd = g.offsets.ptr, g.heads.ptr
+    else:
+        off, hd = g.offsets.ctypes.data, _p(g.heads)
+    pv = None
+    if per_vertex_out is not None:
+        flags |= KT_DEVICE_OUTPUT
+        pv_ptr = per_vertex_out.ptr
+    elif per_vertex is not False and per_vertex is not None:
+        if isinstance(per_vertex, np.ndarray):
+            assert per_vertex.dtype == np.uint64 and per_vertex.size == g.n and per_vertex.flags.c_contiguous
+            pv = per_vertex
+        else:
+            pv = np.zeros(g.n, dtype=np.uint64)
+        pv_ptr = pv.ctypes.data if pv.size else None
+    else:
+        pv_ptr = None
+    so = KtShardOut(0, 0, pv_ptr)
+    ctx.check(lib().kt_count_shard_graph(ctx.handle, g.n, off, hd, flags, algorithm, num_ranks, b.ctypes.data, rank,
+                                         C.byref(so)), "kt_count_shard_graph")
+    return int(so.local_triangles), int(so.type3_triangles), pv
