@@ -91,6 +91,13 @@ int kt_ctx::begin_call(bool counting) {
     return KT_OK;
 }
 
+int kt_ctx::read_u64(const uint64_t* dev, uint64_t* out, const char* what) {
+    int rc = hip_check(hipMemcpyAsync(host_small + 63, dev, 8, hipMemcpyDeviceToHost, stream), what);
+    if (!rc) rc = hip_check(hipStreamSynchronize(stream), what);
+    if (!rc) *out = host_small[63];
+    return rc;
+}
+
 float kt_ctx::elapsed(int a, int b) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) {
@@ -299,7 +306,7 @@ int upload_and_count(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_o
         B.off = user_off;
         B.heads = user_heads;
     }
-    hipEventRecord(c->ev[EV_H2D], st);
+    (void)hipEventRecord(c->ev[EV_H2D], st);
     trace("upload done, launching count pass");
     fill_u64(B.small, 64, 0, st);
     orient_count_launch(n, nnz, B.off, B.heads, B.deg, B.ooff, B.ws_orient, reinterpret_cast<uint32_t*>(B.small),
@@ -324,7 +331,7 @@ int run_orientation(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_of
     if (rc) return rc;
     plain_layout(c->arena, n, nnz, host_in, pv_dev_needed, false, B);
     trace("arena=%p bytes=%llu", c->arena, (unsigned long long)c->arena_bytes);
-    hipEventRecord(c->ev[EV_START], st);
+    (void)hipEventRecord(c->ev[EV_START], st);
     rc = upload_and_count(c, n, nnz, user_off, user_heads, host_in, B);
     if (rc) return rc;
     uint32_t flags = static_cast<uint32_t>(c->host_small[0]);
@@ -380,9 +387,9 @@ int run_orientation(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_of
 int read_nnz(kt_ctx* c, uint32_t n, const uint64_t* offsets, uint32_t flags, uint64_t& nnz) {
     if (flags & KT_DEVICE_INPUT) {
         uint64_t v[2] = {0, 0};
-        int rc = c->hip_check(hipMemcpy(v, offsets, 8, hipMemcpyDeviceToHost), "D2H offsets[0]");
+        int rc = c->read_u64(offsets, v, "D2H offsets[0]");
         if (rc) return rc;
-        rc = c->hip_check(hipMemcpy(v + 1, offsets + n, 8, hipMemcpyDeviceToHost), "D2H offsets[n]");
+        rc = c->read_u64(offsets + n, v + 1, "D2H offsets[n]");
         if (rc) return rc;
         if (v[0] != 0) return c->fail(KT_EINVAL, "offsets[0] != 0");
         nnz = v[1];
@@ -418,7 +425,7 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     rc = run_orientation(c, n, nnz, offsets, heads, host_in, want_pv && !pv_dev, B, unsorted);
     if (rc) return rc;
     hipStream_t st = c->stream;
-    hipEventRecord(c->ev[EV_PREP], st);
+    (void)hipEventRecord(c->ev[EV_PREP], st);
     uint64_t* pv = want_pv ? (pv_dev ? per_vertex_out : B.pv) : nullptr;
     if (pv) fill_u64(pv, n, 0, st);
     uint64_t* totals = B.small + 1;
@@ -429,7 +436,7 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     TcOut out{pv, totals};
     trace("intersection: X=%u pv=%p", n, (void*)pv);
     tc_launch(view, out, c->prm, B.ws_tc, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]);
-    hipEventRecord(c->ev[EV_COUNT], st);
+    (void)hipEventRecord(c->ev[EV_COUNT], st);
     uint64_t* tstats = tc_stats_ptr(B.ws_tc);
     rc = c->hip_check(hipMemcpyAsync(c->host_small + 1, totals, 8, hipMemcpyDeviceToHost, st), "D2H total");
     if (rc) return rc;
@@ -440,7 +447,7 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
                           "D2H per-vertex");
         if (rc) return rc;
     }
-    hipEventRecord(c->ev[EV_END], st);
+    (void)hipEventRecord(c->ev[EV_END], st);
     rc = c->hip_check(hipStreamSynchronize(st), "count");
     if (rc) return rc;
     trace("count done: total=%llu", (unsigned long long)c->host_small[1]);

@@ -157,7 +157,7 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
         nnz = offsets[N];
         if (nnz && !heads) return c->fail(KT_EINVAL, "null heads");
     } else {
-        rc = c->hip_check(hipMemcpy(&nnz, offsets + N, 8, hipMemcpyDeviceToHost), "D2H nnz");
+        rc = c->read_u64(offsets + N, &nnz, "D2H nnz");
         if (rc) return rc;
     }
     if (nnz & 1) return c->fail(KT_EINVAL, "odd number of CSR entries: not a symmetric graph");
@@ -185,7 +185,7 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
             B.heads = heads;
         }
         uint64_t* pv = pv_dev ? per_vertex : B.pv;
-        hipEventRecord(c->ev[EV_START], st);
+        (void)hipEventRecord(c->ev[EV_START], st);
         if (host_in) {
             rc = c->hip_check(hipMemcpyAsync(const_cast<uint64_t*>(B.off), offsets, (N + 1) * 8, hipMemcpyHostToDevice,
                                              st),
@@ -199,7 +199,7 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
         rc = c->hip_check(hipMemcpyAsync(B.vbase, vertex_base, ((uint64_t)G + 1) * 8, hipMemcpyHostToDevice, st),
                           "H2D vertex_base");
         if (rc) return rc;
-        hipEventRecord(c->ev[EV_H2D], st);
+        (void)hipEventRecord(c->ev[EV_H2D], st);
         if (!sort) {
             const bool same = S->exec && S->G == G && S->N == N && S->nnz == nnz && S->off == B.off &&
                               S->heads == B.heads && S->pv == pv && S->arena == c->arena &&
@@ -230,20 +230,20 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
                 trace("batch: captured pipeline for G=%u N=%llu nnz=%llu", G, (unsigned long long)N,
                       (unsigned long long)nnz);
             }
-            hipEventRecord(c->ev[EV_PREP], st);
+            (void)hipEventRecord(c->ev[EV_PREP], st);
             rc = c->hip_check(hipGraphLaunch(S->exec, st), "hipGraphLaunch");
             if (rc) return rc;
             S->replays++;
         } else {
-            hipEventRecord(c->ev[EV_PREP], st);
+            (void)hipEventRecord(c->ev[EV_PREP], st);
             batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small, true);
         }
-        hipEventRecord(c->ev[EV_COUNT], st);
+        (void)hipEventRecord(c->ev[EV_COUNT], st);
         if (want_pv && !pv_dev) {
             rc = c->hip_check(hipMemcpyAsync(per_vertex, pv, N * 8, hipMemcpyDeviceToHost, st), "D2H per-vertex");
             if (rc) return rc;
         }
-        hipEventRecord(c->ev[EV_END], st);
+        (void)hipEventRecord(c->ev[EV_END], st);
         rc = c->hip_check(hipStreamSynchronize(st), "batch");
         if (rc) return rc;
         const uint32_t fl = (uint32_t)c->host_small[0];

@@ -63,6 +63,8 @@ struct kt_ctx {
     int reserve(uint64_t bytes);
     int begin_call(bool counting);
     float elapsed(int a, int b);
+    // One device word, read in order after the work queued on `stream`.
+    int read_u64(const uint64_t* dev, uint64_t* out, const char* what);
     void destroy_extensions();
 };
 

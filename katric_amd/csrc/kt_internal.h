@@ -21,6 +21,7 @@ uint64_t scan_tmp_words(uint64_t len);
 void scan_u64(const uint64_t* in, uint64_t* out, uint64_t len, uint64_t* tmp, hipStream_t st);
 uint64_t radix_tmp_words(uint64_t len);
 void radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t len, int bits, uint64_t* tmp, hipStream_t st);
+void tile_bounds_launch(uint32_t n, uint64_t nnz, uint64_t tile, const uint64_t* off, uint32_t* bnd, hipStream_t st);
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz);
 void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,
                          uint64_t* ooff, uint64_t* ws, uint32_t* err, hipStream_t st);

@@ -444,6 +444,15 @@ uint64_t orient_ws_words(uint32_t n, uint64_t nnz) {
            std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1));
 }
 
+// Rows of the first and last slot of every `tile`-slot tile of a CSR whose
+// offsets start at 0 (bnd: 2 entries per tile), for tile_rows.
+void tile_bounds_launch(uint32_t n, uint64_t nnz, uint64_t tile, const uint64_t* off, uint32_t* bnd, hipStream_t st) {
+    const uint64_t tiles = (nnz + tile - 1) / tile;
+    if (!tiles || !n) return;
+    fill_u32(bnd, 2 * tiles, 0, st);
+    k_tile_bounds<<<(unsigned)(((uint64_t)n + 255) / 256), 256, 0, st>>>(n, nnz, tile, off, bnd);
+}
+
 // Degrees, validation flags, keep bitmask, d+ per row and its scan into ooff
 // (ooff[n] = kept total), per-tile kept counts and their scan.
 void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,

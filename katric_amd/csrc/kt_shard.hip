@@ -101,38 +101,39 @@ __device__ __forceinline__ bool c_less(const ShardMap& M, const uint32_t* deg, u
     return da < db || (da == db && M.gid(a) < M.gid(b));
 }
 
-__device__ __forceinline__ void wave_push(uint64_t* keys, uint64_t* ctr, bool pred, uint64_t key) {
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return;
-    const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-    uint64_t base = 0;
-    if (lane_id() == leader) base = atomic_add_u64(ctr, (uint64_t)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (pred) keys[base + __popcll(m & lanemask_lt())] = key;
-}
-
-// Owned slots -> (tail, head) keys of N+ lists and (cetric-x) ghost lists.
+// Owned slots -> (tail, head) keys of N+ lists and (cetric-x) ghost lists,
+// appended in any order (they are sorted next): every wave counts its keys
+// over the tile's 16 slot rounds, and one atomic per workgroup reserves the
+// workgroup's output range.
 __global__ void __launch_bounds__(SH_THREADS)
-    k_shard_entries(ShardMap M, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
-                    const uint32_t* __restrict__ deg, int bits, int ghost_lists, uint64_t* __restrict__ keys,
-                    uint64_t* __restrict__ nkeys, uint32_t* __restrict__ err) {
+    k_shard_entries(ShardMap M, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ tbnd,
+                    const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg, int bits, int ghost_lists,
+                    uint64_t* __restrict__ keys, uint64_t* __restrict__ nkeys, uint32_t* __restrict__ err) {
+    constexpr int NW = SH_THREADS / KT_WAVE;
     __shared__ uint32_t marks[SH_TILE];
-    __shared__ uint32_t wmax[SH_THREADS / KT_WAVE];
+    __shared__ uint32_t wmax[NW];
     __shared__ uint32_t bnd[2];
+    __shared__ uint32_t wtot[NW];
+    __shared__ uint64_t bbase;
     const uint64_t tile0 = (uint64_t)blockIdx.x * SH_TILE;
     const uint64_t tend = min(tile0 + SH_TILE, nnz);
     uint32_t lo, hi;
-    tile_rows<SH_THREADS, SH_ITEMS>(off, M.nl, tile0, tend, marks, bnd, wmax, lo, hi);
-    uint32_t flags = 0;
+    tile_rows<SH_THREADS, SH_ITEMS>(off, M.nl, tile0, tend, marks, bnd, wmax, lo, hi, tbnd);
+    const unsigned tid = threadIdx.x, wave = tid / KT_WAVE;
+    uint32_t cs[SH_ITEMS];
+#pragma unroll
     for (int k = 0; k < SH_ITEMS; ++k) {
-        const unsigned i = k * SH_THREADS + threadIdx.x;
-        const uint64_t sl = tile0 + i;
+        const uint64_t sl = tile0 + k * SH_THREADS + tid;
+        cs[k] = sl < tend ? M.map(heads[sl]) : 0u;
+    }
+    uint32_t flags = 0, count = 0;
+    uint64_t key[SH_ITEMS], pm[SH_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SH_ITEMS; ++k) {
+        const unsigned i = k * SH_THREADS + tid;
         bool out_e = false, in_e = false;
-        uint32_t v = 0, c = 0;
-        if (sl < tend) {
-            v = lo + marks[i];
-            const uint32_t u = heads[sl];
-            c = M.map(u);
+        const uint32_t v = lo + marks[i], c = cs[k];
+        if (tile0 + i < tend) {
             if (c == 0xffffffffu) {
                 flags |= KT_ERR_GHOST;
             } else if (c == v) {
@@ -142,10 +143,26 @@ __global__ void __launch_bounds__(SH_THREADS)
                 in_e = ghost_lists && !out_e && c >= M.nl;
             }
         }
-        wave_push(keys, nkeys, out_e || in_e,
-                  out_e ? (((uint64_t)v << bits) | c) : (((uint64_t)c << bits) | v));
+        key[k] = out_e ? (((uint64_t)v << bits) | c) : (((uint64_t)c << bits) | v);
+        pm[k] = __ballot(out_e || in_e);
+        count += (uint32_t)__popcll(pm[k]);
     }
     if (flags) atomicOr(err, flags);
+    if (lane_id() == 0) wtot[wave] = count;
+    __syncthreads();
+    if (tid == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < NW; ++w) t += wtot[w];
+        bbase = t ? atomic_add_u64(nkeys, t) : 0;
+    }
+    __syncthreads();
+    uint64_t base = bbase;
+    for (unsigned w = 0; w < wave; ++w) base += wtot[w];
+#pragma unroll
+    for (int k = 0; k < SH_ITEMS; ++k) {
+        if ((pm[k] >> lane_id()) & 1) keys[base + __popcll(pm[k] & lanemask_lt())] = key[k];
+        base += __popcll(pm[k]);
+    }
 }
 
 // Sorted keys -> heads of the CSR over X tails, and the list length of every
@@ -193,36 +210,48 @@ __global__ void k_cut_begin(uint32_t nl, const uint64_t* __restrict__ coff, cons
     cut[u] = lo;
 }
 
+// Messages are walked by groups of MSG_G lanes (one message per group; most
+// messages are short), with ballots masked to the group.
+constexpr unsigned MSG_G = 16;
+
+__device__ __forceinline__ uint64_t group_mask() { return 0xffffull << (lane_id() & ~(MSG_G - 1)); }
+
 // Per message: owned / ghost entry counts; sender -> combined index.
 __global__ void k_msg_count(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint32_t* __restrict__ src,
                             const uint64_t* __restrict__ moff, const uint32_t* __restrict__ mheads,
                             uint64_t* __restrict__ kept, uint32_t* __restrict__ nown, uint32_t* __restrict__ tail_id,
                             uint32_t* __restrict__ err) {
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KT_WAVE;
-    const uint32_t nw = gridDim.x * blockDim.x / KT_WAVE;
-    for (uint32_t i = wid; i < nmsg; i += nw) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / MSG_G;
+    const unsigned gl = lane_id() & (MSG_G - 1);
+    const uint64_t gmask = group_mask();
+    uint32_t flags = 0;
+    for (uint64_t i = t / MSG_G; i < nmsg; i += groups) {
         const uint64_t b = moff[i], e = moff[i + 1];
+        const bool bad = e < b || e > mtot;
         uint32_t own = 0, gh = 0;
-        if (e < b || e > mtot) {
-            if (lane_id() == 0) atomicOr(err, KT_ERR_MESSAGE);
-        } else {
-            for (uint64_t p = b + lane_id(); p < e; p += KT_WAVE) {
-                const uint32_t c = M.map(mheads[p]);
-                own += (c < M.nl) ? 1u : 0u;
-                gh += (c != 0xffffffffu && c >= M.nl) ? 1u : 0u;
-                if (p + 1 < e && mheads[p + 1] <= mheads[p]) atomicOr(err, KT_ERR_MESSAGE);
+        if (!bad) {
+            for (uint64_t p0 = b; p0 < e; p0 += MSG_G) {  // uniform over the group
+                const uint64_t p = p0 + gl;
+                uint32_t c = 0xffffffffu;
+                if (p < e) {
+                    const uint32_t x = mheads[p];
+                    c = M.map(x);
+                    if (p + 1 < e && mheads[p + 1] <= x) flags |= KT_ERR_MESSAGE;
+                }
+                own += (uint32_t)__popcll(__ballot(c < M.nl) & gmask);
+                gh += (uint32_t)__popcll(__ballot(c != 0xffffffffu && c >= M.nl) & gmask);
             }
         }
-        own = wave_sum(own);
-        gh = wave_sum(gh);
-        if (lane_id() == 0) {
+        if (gl == 0) {
             const uint32_t a = M.map(src[i]);
-            if (a == 0xffffffffu || a < M.nl) atomicOr(err, KT_ERR_MESSAGE);
+            if (bad || a == 0xffffffffu || a < M.nl) flags |= KT_ERR_MESSAGE;
             tail_id[i] = (a == 0xffffffffu) ? 0u : a;
             kept[i] = own + gh;
             nown[i] = own;
         }
     }
+    if (flags) atomicOr(err, flags);
 }
 
 // Per message: owned entries then ghost entries, each in message order.
@@ -230,20 +259,22 @@ __global__ void k_msg_write(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint
                             const uint32_t* __restrict__ mheads, const uint64_t* __restrict__ coff_msg,
                             const uint32_t* __restrict__ nown, uint32_t* __restrict__ out,
                             uint64_t* __restrict__ split) {
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / KT_WAVE;
-    const uint32_t nw = gridDim.x * blockDim.x / KT_WAVE;
-    for (uint32_t i = wid; i < nmsg; i += nw) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / MSG_G;
+    const unsigned gl = lane_id() & (MSG_G - 1);
+    const uint64_t gmask = group_mask();
+    for (uint64_t i = t / MSG_G; i < nmsg; i += groups) {
         const uint64_t b = moff[i], e = moff[i + 1];
         uint64_t po = coff_msg[i], pg = coff_msg[i] + nown[i];
-        if (lane_id() == 0) split[i] = pg;
+        if (gl == 0) split[i] = pg;
         if (e < b || e > mtot) continue;  // flagged by k_msg_count; nothing was counted for it
-        for (uint64_t p0 = b; p0 < e; p0 += KT_WAVE) {
-            const uint64_t p = p0 + lane_id();
+        for (uint64_t p0 = b; p0 < e; p0 += MSG_G) {
+            const uint64_t p = p0 + gl;
             uint32_t c = 0xffffffffu;
             if (p < e) c = M.map(mheads[p]);
             const bool own = c < M.nl;
             const bool gh = c != 0xffffffffu && c >= M.nl;
-            const uint64_t mo = __ballot(own), mg = __ballot(gh);
+            const uint64_t mo = __ballot(own) & gmask, mg = __ballot(gh) & gmask;
             if (own) out[po + __popcll(mo & lanemask_lt())] = c;
             if (gh) out[pg + __popcll(mg & lanemask_lt())] = c;
             po += __popcll(mo);
@@ -290,6 +321,7 @@ struct ShardBufs {
     uint64_t* tc_ws;
     uint64_t* pv;
     uint32_t* dense;
+    uint32_t* tbnd;
 };
 
 // A dense global-id table is used up to 2^30 ids (4 GiB).
@@ -328,6 +360,7 @@ uint64_t shard_layout(void* base, const kt_shard_in* in, uint64_t nnz, uint64_t 
     B.pv = pv_dev ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(X, 1) * 8)) : nullptr;
     const uint64_t n_all = in->range_bounds[in->num_ranks];
     B.dense = n_all <= DENSE_MAX_IDS ? static_cast<uint32_t*>(cv.take(std::max<uint64_t>(n_all, 1) * 4)) : nullptr;
+    B.tbnd = static_cast<uint32_t*>(cv.take((2 * ((nnz + SH_TILE - 1) / SH_TILE) + 1) * 4));
     return cv.used;
 }
 
@@ -361,10 +394,10 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
         mtot = in->n_msg ? in->msg_offsets[in->n_msg] : 0;
         if (in->n_msg && in->msg_offsets[0] != 0) return c->fail(KT_EINVAL, "msg_offsets[0] != 0");
     } else {
-        rc = c->hip_check(hipMemcpy(&nnz, in->offsets + nl, 8, hipMemcpyDeviceToHost), "D2H nnz");
+        rc = c->read_u64(in->offsets + nl, &nnz, "D2H nnz");
         if (rc) return rc;
         if (in->n_msg) {
-            rc = c->hip_check(hipMemcpy(&mtot, in->msg_offsets + in->n_msg, 8, hipMemcpyDeviceToHost), "D2H mtot");
+            rc = c->read_u64(in->msg_offsets + in->n_msg, &mtot, "D2H mtot");
             if (rc) return rc;
         }
     }
@@ -377,7 +410,7 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
     if (rc) return rc;
     shard_layout(c->arena, in, nnz, mtot, host_in, want_pv && !pv_dev, B);
     hipStream_t st = c->stream;
-    hipEventRecord(c->ev[EV_START], st);
+    (void)hipEventRecord(c->ev[EV_START], st);
     if (host_in) {
         struct {
             const void* dst;
@@ -405,7 +438,7 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
         B.msg_off = in->msg_offsets;
         B.msg_heads = in->msg_heads;
     }
-    hipEventRecord(c->ev[EV_H2D], st);
+    (void)hipEventRecord(c->ev[EV_H2D], st);
     uint32_t* err = reinterpret_cast<uint32_t*>(B.small);
     uint64_t* nkeys = B.small + 1;
     fill_u64(B.small, 64, 0, st);
@@ -422,9 +455,11 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
     k_shard_prepare<<<(unsigned)std::min<uint64_t>((X + 255) / 256, 8192), 256, 0, st>>>(M, nnz, B.off,
                                                                                       B.ghost_degree, B.deg, err);
     const uint64_t tiles = (nnz + SH_TILE - 1) / SH_TILE;
-    if (tiles && nl)
-        k_shard_entries<<<(unsigned)tiles, SH_THREADS, 0, st>>>(M, nnz, B.off, B.heads, B.deg, bits, cetric ? 1 : 0,
-                                                                B.keys, nkeys, err);
+    if (tiles && nl) {
+        tile_bounds_launch(nl, nnz, SH_TILE, B.off, B.tbnd, st);
+        k_shard_entries<<<(unsigned)tiles, SH_THREADS, 0, st>>>(M, nnz, B.off, B.tbnd, B.heads, B.deg, bits,
+                                                                cetric ? 1 : 0, B.keys, nkeys, err);
+    }
     rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 16, hipMemcpyDeviceToHost, st), "D2H flags");
     if (rc) return rc;
     rc = c->hip_check(hipStreamSynchronize(st), "shard entries");
@@ -444,10 +479,10 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
     if (nl) k_cut_begin<<<(nl + 255) / 256, 256, 0, st>>>(nl, B.coff, B.cheads, B.cut);
     // messages
     if (in->n_msg) {
-        k_msg_count<<<(unsigned)std::min<uint64_t>(((uint64_t)in->n_msg * 64 + 255) / 256, 8192), 256, 0, st>>>(
+        k_msg_count<<<(unsigned)std::min<uint64_t>(((uint64_t)in->n_msg * MSG_G + 255) / 256, 8192), 256, 0, st>>>(
             M, in->n_msg, mtot, B.msg_src, B.msg_off, B.msg_heads, B.mkept, B.mnown, B.mtail, err);
         scan_u64(B.mkept, B.moff, in->n_msg, B.scan_tmp, st);
-        k_msg_write<<<(unsigned)std::min<uint64_t>(((uint64_t)in->n_msg * 64 + 255) / 256, 8192), 256, 0, st>>>(
+        k_msg_write<<<(unsigned)std::min<uint64_t>(((uint64_t)in->n_msg * MSG_G + 255) / 256, 8192), 256, 0, st>>>(
             M, in->n_msg, mtot, B.msg_off, B.msg_heads, B.moff, B.mnown, B.mheads2, B.msplit);
     }
     rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st), "D2H flags");
@@ -458,7 +493,7 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
     if (flags & KT_ERR_MASK)
         return c->fail(KT_EINVAL, "malformed shard:%s%s", (flags & KT_ERR_DUPLICATE) ? " duplicate edge" : "",
                        (flags & KT_ERR_MESSAGE) ? " bad message" : "");
-    hipEventRecord(c->ev[EV_PREP], st);
+    (void)hipEventRecord(c->ev[EV_PREP], st);
     uint64_t* pv = want_pv ? (pv_dev ? out->per_vertex : B.pv) : nullptr;
     if (pv) fill_u64(pv, X, 0, st);
     uint64_t* totals = B.small + 4;  // [4] local, [5] type3
@@ -508,14 +543,14 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
         rc = c->hip_check(hipMemcpyAsync(c->host_small + 12, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
         if (rc) return rc;
     }
-    hipEventRecord(c->ev[EV_COUNT], st);
+    (void)hipEventRecord(c->ev[EV_COUNT], st);
     rc = c->hip_check(hipMemcpyAsync(c->host_small + 4, totals, 16, hipMemcpyDeviceToHost, st), "D2H totals");
     if (rc) return rc;
     if (want_pv && !pv_dev) {
         rc = c->hip_check(hipMemcpyAsync(out->per_vertex, pv, X * 8, hipMemcpyDeviceToHost, st), "D2H per-vertex");
         if (rc) return rc;
     }
-    hipEventRecord(c->ev[EV_END], st);
+    (void)hipEventRecord(c->ev[EV_END], st);
     rc = c->hip_check(hipStreamSynchronize(st), "shard count");
     if (rc) return rc;
     out->local_triangles = c->host_small[4];

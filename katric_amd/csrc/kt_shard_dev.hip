@@ -12,11 +12,16 @@
 //   3. both bitmaps compacted in id order (popcount per word, scan, write):
 //      ghost ids ascending with their global degrees (DegreeCommunicator,
 //      graph_communicator.h:48-266), senders ascending;
-//   4. per sender a, a group of 16 lanes walks a's full list and keeps the x
-//      with (deg a, a) < (deg x, x); cetric-x also drops the x owned by a's
-//      owner (N+cut(a)), patric keeps all of N+(a).  Kept ids are counted, the
-//      counts scanned and the ids written in list order; lists that are not
-//      ascending are put in order by one key sort over (message, id).
+//   4. the senders' full lists, concatenated in sender order, form a virtual
+//      slot space (virtual offsets = scan of the senders' degrees); tiles of
+//      4096 virtual slots resolve each slot's sender in LDS and keep x with
+//      (deg a, a) < (deg x, x) -- cetric-x also drops the x owned by a's owner
+//      (N+cut(a)), patric keeps all of N+(a).  A counting pass gives each
+//      tile's kept count; after their scan the same pass writes every kept id
+//      at (tile base + rank in the tile), which is its position in the
+//      concatenated message lists, and each sender's first virtual slot gives
+//      its message offset.  Lists that are not ascending are put in order by
+//      one key sort over (message, id).
 // The rank's slice is not copied: its offsets are rebased into a new array and
 // its heads are the full graph's heads from off[from] on.  The senders are
 // found from the rank's own slots; for a symmetric CSR that is the same set
@@ -32,7 +37,6 @@ namespace kt {
 constexpr int PD_THREADS = 256;
 constexpr int PD_ITEMS = 16;
 constexpr int PD_TILE = PD_THREADS * PD_ITEMS;
-constexpr int PD_GROUP = 16;  // lanes per sender list walk
 
 __device__ __forceinline__ bool before(uint32_t da, uint32_t a, uint32_t db, uint32_t b) {
     return da < db || (da == db && a < b);
@@ -58,29 +62,37 @@ __global__ void k_pd_degree(uint32_t n, uint64_t nnz, const uint64_t* __restrict
     }
 }
 
-// Slots [base, end) of the rows [from, to); soff = off + from.
+// Slots of the rank's rows [from, from + nl): loff = rebased offsets (loff[0]
+// = 0), lheads = the rows' heads, tbnd = rows of every tile's first and last
+// slot.
 __global__ void __launch_bounds__(PD_THREADS)
-    k_pd_mark(const uint64_t* __restrict__ soff, uint32_t nl, uint32_t from, uint32_t n, uint64_t base, uint64_t end,
-              const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg, uint32_t* __restrict__ gbits,
-              uint32_t* __restrict__ sbits, uint32_t* __restrict__ err) {
+    k_pd_mark(const uint64_t* __restrict__ loff, uint32_t nl, uint32_t from, uint32_t n, uint64_t nnz_l,
+              const uint32_t* __restrict__ lheads, const uint32_t* __restrict__ tbnd, const uint32_t* __restrict__ deg,
+              uint32_t* __restrict__ gbits, uint32_t* __restrict__ sbits, uint32_t* __restrict__ err) {
     __shared__ uint32_t marks[PD_TILE];
     __shared__ uint32_t wmax[PD_THREADS / KT_WAVE];
     __shared__ uint32_t bnd[2];
-    const uint64_t tile0 = base + (uint64_t)blockIdx.x * PD_TILE;
-    const uint64_t tend = min(tile0 + PD_TILE, end);
+    const uint64_t tile0 = (uint64_t)blockIdx.x * PD_TILE;
+    const uint64_t tend = min(tile0 + PD_TILE, nnz_l);
     uint32_t lo, hi;
-    tile_rows<PD_THREADS, PD_ITEMS>(soff, nl, tile0, tend, marks, bnd, wmax, lo, hi);
+    tile_rows<PD_THREADS, PD_ITEMS>(loff, nl, tile0, tend, marks, bnd, wmax, lo, hi, tbnd);
     const uint32_t to = from + nl;
+    uint32_t us[PD_ITEMS];
+#pragma unroll
+    for (int k = 0; k < PD_ITEMS; ++k) {
+        const uint64_t sl = tile0 + k * PD_THREADS + threadIdx.x;
+        us[k] = sl < tend ? lheads[sl] : from;
+    }
     uint32_t flags = 0;
+#pragma unroll
     for (int k = 0; k < PD_ITEMS; ++k) {
         const unsigned i = k * PD_THREADS + threadIdx.x;
-        const uint64_t sl = tile0 + i;
-        if (sl >= tend) continue;
-        const uint32_t x = from + lo + marks[i];
-        const uint32_t u = heads[sl];
+        const uint32_t u = us[k];
+        if (tile0 + i >= tend) continue;
         if (u >= n) {
             flags |= KT_ERR_HEAD_RANGE;
         } else if (u < from || u >= to) {
+            const uint32_t x = from + lo + marks[i];
             set_bit(gbits, u);
             if (before(deg[u], u, deg[x], x)) set_bit(sbits, u);
         }
@@ -100,7 +112,8 @@ __global__ void k_pd_popc(uint64_t words, const uint32_t* __restrict__ gbits, co
 __global__ void k_pd_compact(uint64_t words, const uint32_t* __restrict__ gbits, const uint64_t* __restrict__ goff,
                              const uint32_t* __restrict__ sbits, const uint64_t* __restrict__ soff,
                              const uint64_t* __restrict__ off, uint32_t* __restrict__ ghost_ids,
-                             uint64_t* __restrict__ ghost_deg, uint32_t* __restrict__ msg_src) {
+                             uint64_t* __restrict__ ghost_deg, uint32_t* __restrict__ msg_src,
+                             uint64_t* __restrict__ vlen) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
         uint32_t bits = gbits[w];
@@ -115,8 +128,11 @@ __global__ void k_pd_compact(uint64_t words, const uint32_t* __restrict__ gbits,
         bits = sbits[w];
         p = soff[w];
         while (bits) {
-            msg_src[p++] = (uint32_t)(w * 32) + (__ffs(bits) - 1);
+            const uint32_t id = (uint32_t)(w * 32) + (__ffs(bits) - 1);
             bits &= bits - 1;
+            msg_src[p] = id;
+            vlen[p] = off[id + 1] - off[id];
+            ++p;
         }
     }
 }
@@ -143,68 +159,114 @@ __device__ __forceinline__ void owner_range(const uint64_t* __restrict__ bd, uin
     hi = bd[o + 1];
 }
 
-// One 16-lane group walks sender a's full list: `kept` receives the number of
-// ids its message keeps; WRITE also stores them from out[pos] on, in list
-// order.  Without WRITE, a list that is not strictly ascending sets
-// KT_FLAG_UNSORTED in `flags`.
-template <bool WRITE>
-__device__ __forceinline__ void walk_sender(uint32_t a, const uint64_t* __restrict__ off,
-                                            const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg,
-                                            uint32_t n, const uint64_t* __restrict__ bd, uint32_t P, bool cetric,
-                                            uint64_t pos, uint32_t* __restrict__ out, uint64_t& kept,
-                                            uint32_t& flags) {
-    const unsigned lane = lane_id();
-    const unsigned gl = lane & (PD_GROUP - 1);
-    const uint64_t gmask = 0xffffull << (lane & ~(PD_GROUP - 1));
-    const uint64_t b = off[a], e = off[a + 1];
-    const uint32_t da = deg[a];
-    uint64_t lo = 0, hi = 0;  // patric: nothing excluded
-    if (cetric) owner_range(bd, P, a, lo, hi);
-    kept = 0;
-    for (uint64_t p0 = b; p0 < e; p0 += PD_GROUP) {  // uniform over the group
-        const uint64_t p = p0 + gl;
+// What a virtual slot needs of its sender.
+struct SenderInfo {
+    uint64_t offa;    // off[a]: a's first slot in the full CSR
+    uint64_t vbeg;    // the message's first virtual slot
+    uint32_t a, da;   // sender id and its global degree
+    uint32_t lo, hi;  // ids excluded from the message: cetric-x those owned by a's owner, patric none
+};
+
+__global__ void k_pd_sinfo(uint32_t nmsg, const uint32_t* __restrict__ src, const uint64_t* __restrict__ voff,
+                           const uint64_t* __restrict__ off, const uint32_t* __restrict__ deg,
+                           const uint64_t* __restrict__ bd, uint32_t P, int cetric, SenderInfo* __restrict__ si) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nmsg; i += gridDim.x * blockDim.x) {
+        const uint32_t a = src[i];
+        uint64_t lo = 0, hi = 0;
+        if (cetric) owner_range(bd, P, a, lo, hi);
+        SenderInfo s;
+        s.offa = off[a];
+        s.vbeg = voff[i];
+        s.a = a;
+        s.da = deg[a];
+        s.lo = (uint32_t)lo;
+        s.hi = (uint32_t)hi;
+        si[i] = s;
+    }
+}
+
+// Tiles of the virtual slot space [0, V).  Counting pass (!EMIT): per-tile
+// kept counts, validation and order flags.  EMIT: kept ids to mheads at
+// tile_off[tile] + rank in the tile, and moff[i] = the output position of
+// sender i's first virtual slot.
+template <bool EMIT>
+__global__ void __launch_bounds__(PD_THREADS)
+    k_pd_msgs(uint64_t V, uint32_t nmsg, const uint64_t* __restrict__ voff, const uint32_t* __restrict__ tbnd,
+              const SenderInfo* __restrict__ si, const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg,
+              uint32_t n, uint64_t* __restrict__ tile_cnt, const uint64_t* __restrict__ tile_off,
+              uint32_t* __restrict__ mheads, uint64_t* __restrict__ moff, uint32_t* __restrict__ err) {
+    constexpr int NW = PD_THREADS / KT_WAVE;
+    __shared__ uint32_t marks[PD_TILE];
+    __shared__ uint32_t wmax[NW];
+    __shared__ uint32_t bnd[2];
+    __shared__ uint32_t wcnt[PD_ITEMS * NW];
+    const uint64_t tile0 = (uint64_t)blockIdx.x * PD_TILE;
+    const uint64_t tend = min(tile0 + PD_TILE, V);
+    uint32_t lo, hi;
+    tile_rows<PD_THREADS, PD_ITEMS>(voff, nmsg, tile0, tend, marks, bnd, wmax, lo, hi, tbnd);
+    const unsigned tid = threadIdx.x, wave = tid / KT_WAVE;
+    // every item's list slot and head first, then the degree gathers
+    uint32_t xs[PD_ITEMS], dx[PD_ITEMS];
+    uint64_t ps[PD_ITEMS];
+#pragma unroll
+    for (int k = 0; k < PD_ITEMS; ++k) {
+        const unsigned idx = k * PD_THREADS + tid;
+        const uint64_t j = tile0 + idx;
+        const SenderInfo* s = si + (lo + marks[idx]);
+        ps[k] = s->offa + (j - s->vbeg);
+        xs[k] = j < tend ? heads[ps[k]] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < PD_ITEMS; ++k) dx[k] = (tile0 + k * PD_THREADS + tid < tend && xs[k] < n) ? deg[xs[k]] : 0u;
+    uint32_t flags = 0;
+    uint64_t bits[PD_ITEMS];
+#pragma unroll
+    for (int k = 0; k < PD_ITEMS; ++k) {
+        const unsigned idx = k * PD_THREADS + tid;
+        const uint64_t j = tile0 + idx;
         bool keep = false;
-        uint32_t x = 0;
-        if (p < e) {
-            x = heads[p];
+        if (j < tend) {
+            const SenderInfo* s = si + (lo + marks[idx]);
+            const uint32_t a = s->a, x = xs[k];
             if (x >= n) {
                 flags |= KT_ERR_HEAD_RANGE;
             } else {
-                keep = before(da, a, deg[x], x) && !(x >= lo && x < hi);
-                if (!WRITE && p + 1 < e && heads[p + 1] <= x) flags |= KT_FLAG_UNSORTED;
+                keep = before(s->da, a, dx[k], x) && !(x >= s->lo && x < s->hi);
+                if (!EMIT && ps[k] + 1 < s->offa + s->da && heads[ps[k] + 1] <= x) flags |= KT_FLAG_UNSORTED;
             }
         }
-        const uint64_t m = __ballot(keep) & gmask;
-        if (WRITE && keep) out[pos + kept + __popcll(m & lanemask_lt())] = x;
-        kept += __popcll(m);
+        bits[k] = __ballot(keep);
+        if (lane_id() == 0) wcnt[k * NW + wave] = (uint32_t)__popcll(bits[k]);
     }
-}
-
-__global__ void k_pd_msg_len(uint32_t nmsg, const uint32_t* __restrict__ src, const uint64_t* __restrict__ off,
-                             const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg, uint32_t n,
-                             const uint64_t* __restrict__ bd, uint32_t P, int cetric, uint64_t* __restrict__ len,
-                             uint32_t* __restrict__ err) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / PD_GROUP;
-    uint32_t flags = 0;
-    for (uint64_t i = t / PD_GROUP; i < nmsg; i += groups) {
-        uint64_t kept;
-        walk_sender<false>(src[i], off, heads, deg, n, bd, P, cetric != 0, 0, nullptr, kept, flags);
-        if ((t & (PD_GROUP - 1)) == 0) len[i] = kept;
+    if (!EMIT && flags) atomicOr(err, flags);
+    __syncthreads();
+    if (!EMIT) {
+        if (tid == 0) {
+            uint64_t t = 0;
+            for (int q = 0; q < PD_ITEMS * NW; ++q) t += wcnt[q];
+            tile_cnt[blockIdx.x] = t;
+        }
+        return;
     }
-    if (flags) atomicOr(err, flags);
-}
-
-__global__ void k_pd_msg_fill(uint32_t nmsg, const uint32_t* __restrict__ src, const uint64_t* __restrict__ off,
-                              const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg, uint32_t n,
-                              const uint64_t* __restrict__ bd, uint32_t P, int cetric,
-                              const uint64_t* __restrict__ moff, uint32_t* __restrict__ mheads) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / PD_GROUP;
-    uint32_t flags = 0;
-    for (uint64_t i = t / PD_GROUP; i < nmsg; i += groups) {
-        uint64_t kept;
-        walk_sender<true>(src[i], off, heads, deg, n, bd, P, cetric != 0, moff[i], mheads, kept, flags);
+    uint64_t base = tile_off[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < PD_ITEMS; ++k) {
+        uint32_t before_w = 0, round = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = wcnt[k * NW + w];
+            before_w += (w < (int)wave) ? c : 0u;
+            round += c;
+        }
+        const unsigned idx = k * PD_THREADS + tid;
+        const uint64_t j = tile0 + idx;
+        const uint64_t pos = base + before_w + __popcll(bits[k] & lanemask_lt());
+        if ((bits[k] >> lane_id()) & 1) mheads[pos] = xs[k];
+        if (j < tend) {
+            const uint32_t i = lo + marks[idx];
+            if (j == si[i].vbeg) moff[i] = pos;
+        }
+        base += round;
     }
 }
 
@@ -288,11 +350,15 @@ struct ProdBufs {
     uint64_t* gcnt;
     uint64_t* scnt;
     uint64_t* stmp;
+    uint32_t* tbnd;
+    uint64_t* tcnt;  // per-tile kept counts of the virtual slot space, then their scan
     uint64_t* loff;
     uint32_t* ghost_ids;
     uint64_t* ghost_deg;
     uint32_t* msg_src;
+    uint64_t* voff;  // virtual offsets of the senders' full lists
     uint64_t* moff;
+    SenderInfo* sinfo;
     uint64_t* pvx;
     uint64_t* gpv;
     uint64_t* hoff;
@@ -303,18 +369,23 @@ uint64_t prod_layout(void* base, uint32_t n, uint64_t nnz, uint32_t P, uint32_t 
                      bool host_pv, bool host_in, ProdBufs& B) {
     Carver cv{base, 0};
     const uint64_t W = ((uint64_t)n + 31) / 32;
+    const uint64_t tiles = nnz / PD_TILE + 1;  // bounds the tiles of the slice and of the virtual slot space
     B.small = static_cast<uint64_t*>(cv.take(64 * 8));
     B.bd = static_cast<uint64_t*>(cv.take(((uint64_t)P + 1) * 8));
     B.deg = static_cast<uint32_t*>(cv.take((uint64_t)n * 4));
     B.bits = static_cast<uint32_t*>(cv.take(2 * W * 4));
     B.gcnt = static_cast<uint64_t*>(cv.take((W + 1) * 8));
     B.scnt = static_cast<uint64_t*>(cv.take((W + 1) * 8));
-    B.stmp = static_cast<uint64_t*>(cv.take(scan_tmp_words(std::max<uint64_t>(W, ng_max) + 1) * 8));
+    B.stmp = static_cast<uint64_t*>(cv.take(scan_tmp_words(std::max(std::max<uint64_t>(W, ng_max), tiles) + 1) * 8));
+    B.tbnd = static_cast<uint32_t*>(cv.take(2 * tiles * 4));
+    B.tcnt = static_cast<uint64_t*>(cv.take((tiles + 1) * 8));
     B.loff = static_cast<uint64_t*>(cv.take(((uint64_t)nl + 1) * 8));
     B.ghost_ids = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(ng_max, 1) * 4));
     B.ghost_deg = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(ng_max, 1) * 8));
     B.msg_src = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(ng_max, 1) * 4));
+    B.voff = static_cast<uint64_t*>(cv.take((ng_max + 1) * 8));
     B.moff = static_cast<uint64_t*>(cv.take((ng_max + 1) * 8));
+    B.sinfo = static_cast<SenderInfo*>(cv.take(std::max<uint64_t>(ng_max, 1) * sizeof(SenderInfo)));
     B.pvx = want_pv ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(nl + ng_max, 1) * 8)) : nullptr;
     B.gpv = host_pv ? static_cast<uint64_t*>(cv.take((uint64_t)n * 8)) : nullptr;
     B.hoff = host_in ? static_cast<uint64_t*>(cv.take(((uint64_t)n + 1) * 8)) : nullptr;
@@ -322,7 +393,7 @@ uint64_t prod_layout(void* base, uint32_t n, uint64_t nnz, uint32_t P, uint32_t 
     return cv.used;
 }
 
-int read_flags(kt_ctx* c, const uint64_t* small, int words, const char* what) {
+int read_small(kt_ctx* c, const uint64_t* small, int words, const char* what) {
     int rc = c->hip_check(hipMemcpyAsync(c->host_small, small, words * 8, hipMemcpyDeviceToHost, c->stream), what);
     if (rc) return rc;
     return c->hip_check(hipStreamSynchronize(c->stream), what);
@@ -388,39 +459,50 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
     const uint64_t W = ((uint64_t)n + 31) / 32;
     fill_u64(B.small, 64, 0, st);
     fill_u32(B.bits, 2 * W, 0, st);
+    fill_u64(B.voff, ng_max + 1, 0, st);
     k_pd_degree<<<grid_for(n), 256, 0, st>>>(n, nnz, off, B.deg, err);
-    rc = read_flags(c, B.small, 1, "degrees");
-    if (rc) return rc;
-    if (c->host_small[0] & KT_ERR_MASK) return c->fail(KT_EINVAL, "malformed CSR: bad offsets");
-    // ghosts and senders
-    const uint64_t tiles = (nnz_l + PD_TILE - 1) / PD_TILE;
-    if (nl && tiles)
-        k_pd_mark<<<(unsigned)tiles, PD_THREADS, 0, st>>>(off + from, nl, from, n, base, end, hd, B.deg, B.bits,
-                                                          B.bits + W, err);
+    // The kernels up to the first flag check stay in bounds whatever the
+    // offsets hold: slots are bounded by the host-checked slice [base, end),
+    // rows by tile_rows' clamping, ids by n.
+    k_pd_rebase<<<grid_for((uint64_t)nl + 1), 256, 0, st>>>(nl, off + from, B.loff);
+    const uint64_t tiles_l = (nnz_l + PD_TILE - 1) / PD_TILE;
+    if (nl && tiles_l) {
+        tile_bounds_launch(nl, nnz_l, PD_TILE, B.loff, B.tbnd, st);
+        k_pd_mark<<<(unsigned)tiles_l, PD_THREADS, 0, st>>>(B.loff, nl, from, n, nnz_l, hd + base, B.tbnd, B.deg,
+                                                            B.bits, B.bits + W, err);
+    }
+    // ghosts and senders, in id order; the senders' degrees are the lengths
+    // of the virtual lists (entries past the last sender stay zero)
     k_pd_popc<<<grid_for(W), 256, 0, st>>>(W, B.bits, B.bits + W, B.gcnt, B.scnt);
     scan_u64(B.gcnt, B.gcnt, W, B.stmp, st);
     scan_u64(B.scnt, B.scnt, W, B.stmp, st);
     k_pd_compact<<<grid_for(W), 256, 0, st>>>(W, B.bits, B.gcnt, B.bits + W, B.scnt, off, B.ghost_ids, B.ghost_deg,
-                                               B.msg_src);
-    k_pd_rebase<<<grid_for((uint64_t)nl + 1), 256, 0, st>>>(nl, off + from, B.loff);
+                                               B.msg_src, B.voff);
+    scan_u64(B.voff, B.voff, ng_max, B.stmp, st);
     rc = c->hip_check(hipMemcpyAsync(B.small + 1, B.gcnt + W, 8, hipMemcpyDeviceToDevice, st), "D2D counts");
     if (!rc) rc = c->hip_check(hipMemcpyAsync(B.small + 2, B.scnt + W, 8, hipMemcpyDeviceToDevice, st), "D2D counts");
-    if (!rc) rc = read_flags(c, B.small, 3, "ghosts");
+    if (!rc) rc = c->hip_check(hipMemcpyAsync(B.small + 3, B.voff + ng_max, 8, hipMemcpyDeviceToDevice, st), "D2D V");
+    if (!rc) rc = read_small(c, B.small, 4, "ghosts");
     if (rc) return rc;
     uint32_t fl = (uint32_t)c->host_small[0];
-    const uint64_t ng = c->host_small[1], nmsg = c->host_small[2];
-    if (fl & KT_ERR_MASK) return c->fail(KT_EINVAL, "malformed CSR:%s", (fl & KT_ERR_HEAD_RANGE) ? " head id >= n" : "");
-    if (ng > ng_max || nmsg > ng) return c->fail(KT_EINVAL, "shard producer: inconsistent ghost counts");
-    // message lists
+    const uint64_t ng = c->host_small[1], nmsg = c->host_small[2], V = c->host_small[3];
+    if (fl & KT_ERR_MASK)
+        return c->fail(KT_EINVAL, "malformed CSR:%s%s", (fl & KT_ERR_OFFSETS) ? " bad offsets" : "",
+                       (fl & KT_ERR_HEAD_RANGE) ? " head id >= n" : "");
+    if (ng > ng_max || nmsg > ng || V > nnz) return c->fail(KT_EINVAL, "shard producer: inconsistent ghost counts");
+    // message lists: count, scan, emit over the virtual slot space
     uint64_t mtot = 0;
     uint32_t* mheads = nullptr;
     if (nmsg) {
-        const unsigned mgrid = grid_for(nmsg * PD_GROUP, 256, 8192);
-        k_pd_msg_len<<<mgrid, 256, 0, st>>>((uint32_t)nmsg, B.msg_src, off, hd, B.deg, n, B.bd, P, cetric ? 1 : 0,
-                                            B.moff, err);
-        scan_u64(B.moff, B.moff, nmsg, B.stmp, st);
-        rc = c->hip_check(hipMemcpyAsync(B.small + 1, B.moff + nmsg, 8, hipMemcpyDeviceToDevice, st), "D2D total");
-        if (!rc) rc = read_flags(c, B.small, 2, "message lengths");
+        const uint64_t tiles_v = (V + PD_TILE - 1) / PD_TILE;
+        k_pd_sinfo<<<grid_for(nmsg), 256, 0, st>>>((uint32_t)nmsg, B.msg_src, B.voff, off, B.deg, B.bd, P,
+                                                   cetric ? 1 : 0, B.sinfo);
+        tile_bounds_launch((uint32_t)nmsg, V, PD_TILE, B.voff, B.tbnd, st);
+        k_pd_msgs<false><<<(unsigned)tiles_v, PD_THREADS, 0, st>>>(V, (uint32_t)nmsg, B.voff, B.tbnd, B.sinfo, hd,
+                                                                   B.deg, n, B.tcnt, nullptr, nullptr, nullptr, err);
+        scan_u64(B.tcnt, B.tcnt, tiles_v, B.stmp, st);
+        rc = c->hip_check(hipMemcpyAsync(B.small + 1, B.tcnt + tiles_v, 8, hipMemcpyDeviceToDevice, st), "D2D total");
+        if (!rc) rc = read_small(c, B.small, 2, "message lengths");
         if (rc) return rc;
         fl = (uint32_t)c->host_small[0];
         mtot = c->host_small[1];
@@ -438,8 +520,9 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
         if (rc) return rc;
         cv = Carver{c->prod[1], 0};
         mheads = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(mtot, 1) * 4));
-        k_pd_msg_fill<<<mgrid, 256, 0, st>>>((uint32_t)nmsg, B.msg_src, off, hd, B.deg, n, B.bd, P, cetric ? 1 : 0,
-                                             B.moff, mheads);
+        k_pd_msgs<true><<<(unsigned)tiles_v, PD_THREADS, 0, st>>>(V, (uint32_t)nmsg, B.voff, B.tbnd, B.sinfo, hd,
+                                                                  B.deg, n, nullptr, B.tcnt, mheads, B.moff, err);
+        fill_u64(B.moff + nmsg, 1, mtot, st);
         if (unsorted && mtot) {
             uint64_t* keys = static_cast<uint64_t*>(cv.take(mtot * 8));
             uint64_t* alt = static_cast<uint64_t*>(cv.take(mtot * 8));

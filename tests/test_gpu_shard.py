@@ -60,3 +60,17 @@ def test_shard_bad_ghost_table(ctx):
     with pytest.raises(K.KatricError) as e:
         S.count_shard(ctx, sh)
     assert e.value.status == K.KT_EINVAL
+
+
+def test_shard_bad_message(ctx):
+    # a message list out of order is rejected, and the context keeps working
+    g = K.gen_rmat(10, 8, seed=2)
+    sh = S.make_shard(g, 4, 1)
+    lens = np.diff(sh.msg_offsets.astype(np.int64))
+    i = int(np.nonzero(lens >= 2)[0][0])
+    a, b = int(sh.msg_offsets[i]), int(sh.msg_offsets[i + 1])
+    sh.msg_heads[a:b] = sh.msg_heads[a:b][::-1].copy()
+    with pytest.raises(K.KatricError) as e:
+        S.count_shard(ctx, sh)
+    assert e.value.status == K.KT_EINVAL
+    check_rank(ctx, g, 4, 1, S.CETRIC)
