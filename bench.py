@@ -595,6 +595,20 @@ def run_plain(args):
             "sample": f"full RMAT-{args.scale} workload, minimum of {len(runs)} runs (orient + sort + "
                       f"merge-intersection local phase, {thr} OpenMP threads), {cpu_s:.2f} s",
         }
+        # single-rank katric (CeTriC preset at P = 1, one thread) on a bounded
+        # sample: the same generator at 1/16 of the vertices
+        ss = max(args.scale - 4, 4)
+        gs = K.gen_rmat(ss, args.edge_factor, seed=args.seed)
+        ts = time.perf_counter()
+        sl, s3, _, _ = O.rank_sim(gs, 1, 0, O.CETRIC, threads=1, per_vertex=False)
+        one_s = time.perf_counter() - ts
+        if not args.no_check:
+            assert sl + s3 == ctx.count(gs, per_vertex=False)[0], "single-rank CPU count != GPU count on the sample"
+        result["cpu_baseline_single_rank"] = {
+            "value": gs.m / one_s, "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": f"RMAT-{ss} ef{args.edge_factor} seed={args.seed} ({gs.m} edges), single-rank katric "
+                      f"(CeTriC preset, P = 1), one thread, {one_s:.2f} s",
+        }
     print(json.dumps(result), flush=True)
 
 

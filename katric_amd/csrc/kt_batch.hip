@@ -40,22 +40,43 @@ struct BatchState {
     uint64_t replays = 0;  // launches of captured pipelines (cumulative)
 };
 
-// totals[g] = sum of per-vertex counts of graph g / 3
-__global__ void k_seg_total(uint32_t G, const uint64_t* __restrict__ vbase, const uint64_t* __restrict__ pv,
-                            uint64_t* __restrict__ totals, uint32_t* __restrict__ err) {
-    __shared__ uint64_t red[4];
-    for (uint32_t g = blockIdx.x; g < G; g += gridDim.x) {
-        uint64_t s = 0;
-        for (uint64_t v = vbase[g] + threadIdx.x; v < vbase[g + 1]; v += blockDim.x) s += pv[v];
-        s = wave_sum(s);
-        if (lane_id() == 0) red[threadIdx.x / KT_WAVE] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint64_t t = red[0] + red[1] + red[2] + red[3];
-            if (t % 3) atomicOr(err, 0x40u);
-            totals[g] = t / 3;
+// Per-graph totals = (sum of the graph's per-vertex counts) / 3, as a
+// segmented sum that does not depend on how the vertices split into graphs:
+// each wave sums a chunk of SEG_CHUNK consecutive vertices, one overlap with a
+// graph at a time (the first graph found by binary search in vbase), and adds
+// each non-zero partial sum to acc[g] (zeroed before) with one atomic.
+constexpr uint32_t SEG_CHUNK = 512;
+
+__global__ void k_seg_partial(uint32_t G, uint64_t N, const uint64_t* __restrict__ vbase,
+                              const uint64_t* __restrict__ pv, uint64_t* __restrict__ acc) {
+    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / KT_WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / KT_WAVE;
+    for (uint64_t c0 = wid * SEG_CHUNK; c0 < N; c0 += nw * SEG_CHUNK) {
+        const uint64_t c1 = min(c0 + SEG_CHUNK, N);
+        uint32_t lo = 0, hi = G - 1;  // last graph g with vbase[g] <= c0
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo + 1) / 2;
+            if (vbase[mid] <= c0)
+                lo = mid;
+            else
+                hi = mid - 1;
         }
-        __syncthreads();
+        for (uint32_t g = lo; g < G; ++g) {
+            const uint64_t a = max(vbase[g], c0), b = min(vbase[g + 1], c1);
+            if (a >= c1) break;
+            uint64_t s = 0;
+            for (uint64_t v = a + lane_id(); v < b; v += KT_WAVE) s += pv[v];
+            s = wave_sum(s);
+            if (lane_id() == 0 && s) atomic_add_u64(&acc[g], s);
+        }
+    }
+}
+
+__global__ void k_seg_final(uint32_t G, uint64_t* __restrict__ totals, uint32_t* __restrict__ err) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+        const uint64_t t = totals[g];
+        if (t % 3) atomicOr(err, 0x40u);
+        totals[g] = t / 3;
     }
 }
 
@@ -114,7 +135,10 @@ void batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const Batc
     v.sheads = B.oheads;
     TcOut o{pv, B.small + 2};
     tc_launch(v, o, prm, B.ws_tc, num_cus, st, nullptr, nullptr);
-    k_seg_total<<<std::min<uint32_t>(G, 16384), 256, 0, st>>>(G, B.vbase, pv, B.totals, err);
+    fill_u64(B.totals, G, 0, st);
+    k_seg_partial<<<(unsigned)std::clamp<uint64_t>((N + SEG_CHUNK * 4 - 1) / (SEG_CHUNK * 4), 1, 16384), 256, 0, st>>>(
+        G, N, B.vbase, pv, B.totals);
+    k_seg_final<<<std::min<uint32_t>((G + 255) / 256, 1024), 256, 0, st>>>(G, B.totals, err);
     (void)hipMemcpyAsync(h_totals, B.totals, (uint64_t)G * 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(h_small, B.small, 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(h_small + 1, B.ooff + N, 8, hipMemcpyDeviceToHost, st);

@@ -67,6 +67,19 @@ def test_batch_unsorted_and_empty_graphs(ctx):
     assert t1[1] == 0
 
 
+def test_batch_totals_across_chunk_boundaries(ctx):
+    # per-graph totals are summed in 512-vertex chunks: graphs with no vertices,
+    # graphs that end exactly on a chunk boundary and graphs spanning many chunks
+    empty = K.Graph(0, np.zeros(1, np.uint64), np.zeros(0, np.uint32))
+    graphs = [empty, K.gen_rmat(15, 16, seed=7), empty, K.gen_gnm(512, 6000, seed=8), K.gen_gnm(511, 6000, seed=9),
+              empty, K.gen_rmat(9, 16, seed=10), K.gen_gnm(1025, 9000, seed=11), empty]
+    b = KB.from_graphs(graphs)
+    tot, pv = KB.count_batch(ctx, b, per_vertex=True)
+    want = [ctx.count(g)[0] if g.n else 0 for g in graphs]
+    assert tot.tolist() == want
+    assert int(pv.sum()) == 3 * sum(want)
+
+
 def test_batch_invalid(ctx):
     g = K.Graph(3, np.array([0, 1, 1, 2], np.uint64), np.array([1, 1], np.uint32))  # asymmetric
     b = KB.from_graphs([K.gen_gnm(100, 500, seed=1), g])
