@@ -32,10 +32,8 @@ static double now_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-// One request of a device batch: where its CSR is read from (page-locked host
-// memory, read by the GPU across PCIe, or the batch arena when the host
-// arrays had to be copied in first), its place in the batch, and its first
-// work chunk.
+// One request of a device batch: where its CSR was copied to in the batch
+// arena, its place in the batch, and its first work chunk.
 struct GatherDesc {
     const uint64_t* off;
     const uint32_t* heads;
@@ -50,12 +48,11 @@ __host__ __device__ inline uint64_t gather_chunks(uint64_t n, uint64_t nnz) {
     return (n + GATHER_CHUNK - 1) / GATHER_CHUNK + (nnz + GATHER_CHUNK - 1) / GATHER_CHUNK;
 }
 
-// Builds the batch's disjoint-union CSR in one launch: off[vb_g + i] =
-// off_g[i] + eb_g, heads[eb_g + k] = heads_g[k] + vb_g (a head outside its
-// own graph becomes an invalid id, flagged by the orientation pass), off[N] =
-// nnz.  Blocks take chunks of one request's offsets or heads; reads are
-// coalesced across the block, so a page-locked source streams at PCIe rate
-// without one copy command per request.
+// Rebases the copied CSRs in place into the batch's disjoint-union CSR in one
+// launch: off[vb_g + i] = off_g[i] + eb_g, heads[eb_g + k] = heads_g[k] + vb_g
+// (a head outside its own graph becomes an invalid id, flagged by the
+// orientation pass), off[N] = nnz.  Blocks take chunks of one request's
+// offsets or heads (every element is read and written by the same thread).
 __global__ void __launch_bounds__(256) k_serve_gather(uint32_t G, const GatherDesc* __restrict__ d,
                                                       uint64_t chunks, uint64_t* __restrict__ off,
                                                       uint32_t* __restrict__ heads, uint64_t N, uint64_t nnz) {
@@ -91,8 +88,6 @@ struct ServeReq {
     uint64_t nnz;
     const uint64_t* off;
     const uint32_t* heads;
-    const uint64_t* d_off = nullptr;    // device-readable aliases of page-locked
-    const uint32_t* d_heads = nullptr;  // host arrays (nullptr: copy them in)
     uint64_t* pv_out;
     double t_submit = 0, t_launch = 0, t_done = 0;
     int status = KT_ENOTREADY;
@@ -204,18 +199,15 @@ static int launch(ServeState* S, ServeSlot& sl, bool sort) {
         dg.vb = vb[g];
         dg.eb = e;
         dg.c0 = chunks;
-        dg.off = r->d_off;
-        dg.heads = r->d_heads;
-        if (!r->d_off) {  // pageable host memory: copy in, rebase in place
-            if (r->n && hipMemcpyAsync(d_off + vb[g], r->off, (uint64_t)r->n * 8, hipMemcpyHostToDevice, st) != hipSuccess)
-                return KT_EHIP;
-            dg.off = d_off + vb[g];
-        }
-        if (!r->d_heads) {
-            if (r->nnz && hipMemcpyAsync(d_heads + e, r->heads, r->nnz * 4, hipMemcpyHostToDevice, st) != hipSuccess)
-                return KT_EHIP;
-            dg.heads = d_heads + e;
-        }
+        // the request's arrays are copied into their place in the batch
+        // (copy engines: from page-locked memory these run beside the other
+        // slot's kernels) and rebased in place by k_serve_gather
+        if (r->n && hipMemcpyAsync(d_off + vb[g], r->off, (uint64_t)r->n * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+            return KT_EHIP;
+        if (r->nnz && hipMemcpyAsync(d_heads + e, r->heads, r->nnz * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+            return KT_EHIP;
+        dg.off = d_off + vb[g];
+        dg.heads = d_heads + e;
         chunks += gather_chunks(r->n, r->nnz);
         vb[g + 1] = vb[g] + r->n;
         e += r->nnz;
@@ -223,9 +215,7 @@ static int launch(ServeState* S, ServeSlot& sl, bool sort) {
     if (hipMemcpyAsync(B.vbase, vb, ((uint64_t)G + 1) * 8, hipMemcpyHostToDevice, st) != hipSuccess) return KT_EHIP;
     if (hipMemcpyAsync(d_desc, desc, (uint64_t)G * sizeof(GatherDesc), hipMemcpyHostToDevice, st) != hipSuccess)
         return KT_EHIP;
-    // a grid of half the CUs: enough reads in flight to stream across PCIe,
-    // while the other slot's batch keeps counting on the rest of the device
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, (uint64_t)S->num_cus / 2));
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, (uint64_t)S->num_cus * 4));
     k_serve_gather<<<grid, 256, 0, st>>>(G, d_desc, chunks, d_off, d_heads, N, nnz);
     batch_pipeline(st, S->num_cus, S->prm, B, G, N, nnz, B.pv, sl.h_totals, sl.h_small, sort);
     for (uint32_t g = 0; g < G; ++g) {
@@ -461,22 +451,6 @@ extern "C" int kt_serve_stop(kt_ctx* c) {
     return serve_stop_impl(c);
 }
 
-// The device address of page-locked host memory (the GPU reads it across
-// PCIe), or nullptr for pageable memory, which is copied in instead.
-template <class T>
-static const T* device_alias(const T* p) {
-    hipPointerAttribute_t a{};
-    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
-        (void)hipGetLastError();
-        return nullptr;
-    }
-    if (a.type != hipMemoryTypeHost || !a.devicePointer) return nullptr;
-    const char* host = a.hostPointer ? static_cast<const char*>(a.hostPointer) : reinterpret_cast<const char*>(p);
-    const char* q = reinterpret_cast<const char*>(p);
-    if (q < host) return nullptr;
-    return reinterpret_cast<const T*>(static_cast<const char*>(a.devicePointer) + (q - host));
-}
-
 extern "C" int kt_serve_submit(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads,
                                uint64_t* per_vertex_out, uint64_t* ticket) {
     if (!c || !ticket || !offsets) return KT_EINVAL;
@@ -490,8 +464,6 @@ extern "C" int kt_serve_submit(kt_ctx* c, uint32_t n, const uint64_t* offsets, c
     r->nnz = nnz;
     r->off = offsets;
     r->heads = heads;
-    r->d_off = device_alias(offsets);
-    r->d_heads = nnz ? device_alias(heads) : nullptr;
     r->pv_out = per_vertex_out;
     r->t_submit = now_ms();
     {
