@@ -6,7 +6,9 @@
 //
 //   * a "team" -- one wave, or all eight waves of a 512-thread workgroup for
 //     heavier tails -- takes a tail x and builds a linear-probing hash of
-//     S(x) = N+(x) in LDS (load factor <= 1/2, <= 1/4 where the table allows);
+//     S(x) = N+(x) in LDS (load factor <= 1/16 where the table allows, <= 1/2
+//     at the bins' largest sets): few buckets are full, so a probe rarely
+//     has to continue into the next bucket;
 //   * the team walks R(x) = N+(x) one neighbour u per wave at a time (u, and
 //     the bounds of T(u) = N+(u), are wave-uniform scalar loads); the wave
 //     streams T(u) with coalesced 16-byte loads, four ids per lane, the next
@@ -563,7 +565,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     P.pv = out.per_vertex;
     uint32_t slots = 0;
     if (HASH) {
-        uint32_t logS = ceil_log2(4 * ns);
+        uint32_t logS = ceil_log2(16 * ns);  // load <= 1/16 unless capped
         if (logS < MIN_LOGS) logS = MIN_LOGS;
         slots = 1u << logS;
         if (slots > cap_slots) {  // bins guarantee cap_slots >= 2 * ns
