@@ -273,6 +273,7 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
         kk[e] = kb[bk[e]];
     }
     uint32_t nh = 0;
+    bool need[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const bool m0 = kk[e].x == w[e], m1 = kk[e].y == w[e], m2 = kk[e].z == w[e], m3 = kk[e].w == w[e];
@@ -282,9 +283,15 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
             atomicAdd(&cnt[4 * bk[e] + (m0 ? 0u : m1 ? 1u : m2 ? 2u : 3u)], 1u);
             on_hit(e);
         }
-        // rare: a full home bucket without w -- w may sit in a later bucket
-        // (a divergent branch, skipped as a whole when no lane takes it)
-        if (valid[e] && !hit && kk[e].w != KT_EMPTY) {
+        // a full home bucket without w: w may sit in a later bucket
+        need[e] = valid[e] && !hit && kk[e].w != KT_EMPTY;
+    }
+    // rare at the table loads used: one branch for the four ids, skipped as a
+    // whole when no lane needs it
+    if (need[0] || need[1] || need[2] || need[3]) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (!need[e]) continue;
             uint32_t b = bk[e];
             while (true) {
                 b = (b + 1) & bmask;
