@@ -435,7 +435,9 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     view.sheads = B.oheads;
     TcOut out{pv, totals};
     trace("intersection: X=%u pv=%p", n, (void*)pv);
-    tc_launch(view, out, c->prm, B.ws_tc, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]);
+    rc = c->hip_check(tc_launch(view, out, c->prm, B.ws_tc, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]),
+                      "intersection launch");
+    if (rc) return rc;
     (void)hipEventRecord(c->ev[EV_COUNT], st);
     uint64_t* tstats = tc_stats_ptr(B.ws_tc);
     rc = c->hip_check(hipMemcpyAsync(c->host_small + 1, totals, 8, hipMemcpyDeviceToHost, st), "D2H total");

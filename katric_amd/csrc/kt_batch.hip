@@ -116,7 +116,7 @@ static int bits_for(uint64_t x) {
 
 // Enqueues the device pipeline on `st` (no host synchronisation): results land
 // in h_totals (per graph) and h_small ([0] flags, [1] kept edges, [2..3] stats).
-void batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
+hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
                     uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort) {
     uint32_t* err = reinterpret_cast<uint32_t*>(B.small);
     fill_u64(B.small, 64, 0, st);
@@ -134,7 +134,8 @@ void batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const Batc
     v.s_off = B.ooff;
     v.sheads = B.oheads;
     TcOut o{pv, B.small + 2};
-    tc_launch(v, o, prm, B.ws_tc, num_cus, st, nullptr, nullptr);
+    const hipError_t le = tc_launch(v, o, prm, B.ws_tc, num_cus, st, nullptr, nullptr);
+    if (le != hipSuccess) return le;
     fill_u64(B.totals, G, 0, st);
     k_seg_partial<<<(unsigned)std::clamp<uint64_t>((N + SEG_CHUNK * 4 - 1) / (SEG_CHUNK * 4), 1, 16384), 256, 0, st>>>(
         G, N, B.vbase, pv, B.totals);
@@ -143,6 +144,7 @@ void batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const Batc
     (void)hipMemcpyAsync(h_small, B.small, 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(h_small + 1, B.ooff + N, 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(h_small + 2, tc_stats_ptr(B.ws_tc), 16, hipMemcpyDeviceToHost, st);
+    return hipGetLastError();
 }
 
 }  // namespace kt
@@ -235,8 +237,13 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
                 hipGraph_t graph = nullptr;
                 rc = c->hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "begin capture");
                 if (rc) return rc;
-                batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small, false);
+                const hipError_t le = batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals,
+                                                     c->host_small, false);
                 rc = c->hip_check(hipStreamEndCapture(st, &graph), "end capture");
+                if (!rc && le != hipSuccess) {
+                    (void)hipGraphDestroy(graph);
+                    rc = c->hip_check(le, "batch pipeline launch");
+                }
                 if (rc) return rc;
                 rc = c->hip_check(hipGraphInstantiate(&S->exec, graph, nullptr, nullptr, 0), "instantiate");
                 (void)hipGraphDestroy(graph);
@@ -260,7 +267,9 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
             S->replays++;
         } else {
             (void)hipEventRecord(c->ev[EV_PREP], st);
-            batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small, true);
+            rc = c->hip_check(batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small, true),
+                              "batch pipeline launch");
+            if (rc) return rc;
         }
         (void)hipEventRecord(c->ev[EV_COUNT], st);
         if (want_pv && !pv_dev) {

@@ -35,16 +35,6 @@ __device__ __forceinline__ T wave_incl_sum(T x) {
     return x;
 }
 
-__device__ __forceinline__ int wave_incl_max(int x) {
-    unsigned l = lane_id();
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int y = __shfl_up(x, d, 64);
-        if (l >= (unsigned)d) x = max(x, y);
-    }
-    return x;
-}
-
 template <typename T>
 __device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll
@@ -132,27 +122,10 @@ __device__ __forceinline__ void tile_rows(const uint64_t* __restrict__ off, uint
     __syncthreads();
 }
 
-// Inclusive max-scan across the wave with DPP row shifts and row broadcasts
-// (no LDS traffic): within each 16-lane row by shifts of 1, 2, 4, 8 (lanes
-// shifted in from outside the row read 0, the identity of max over unsigned
-// values), then lane 15 of rows 0 and 2 into rows 1 and 3, then lane 31 into
-// rows 2 and 3.
-__device__ __forceinline__ uint32_t wave_incl_max_dpp(uint32_t v) {
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true));  // row_shr:1
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true));  // row_shr:2
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true));  // row_shr:4
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true));  // row_shr:8
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false)); // row_bcast:15
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false)); // row_bcast:31
-    return v;
-}
-
-// Maximum over the wave (lane 63 of the inclusive max-scan).
-__device__ __forceinline__ uint32_t wave_max_u32_dpp(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_max_dpp(v), 63);
-}
-
-// Sum over the wave (the same DPP pattern as wave_incl_max_dpp, then lane 63).
+// Sum over the wave with DPP row shifts and row broadcasts (no LDS traffic):
+// an inclusive scan within each 16-lane row by shifts of 1, 2, 4, 8 (lanes
+// shifted in from outside the row read 0), then lane 15 of rows 0 and 2 into
+// rows 1 and 3, then lane 31 into rows 2 and 3; lane 63 holds the total.
 __device__ __forceinline__ uint32_t wave_total_dpp(uint32_t v) {
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);  // row_shr:1
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);  // row_shr:2

@@ -103,11 +103,12 @@ struct BatchBufs {
 };
 uint64_t batch_layout(void* base, uint32_t G, uint64_t N, uint64_t nnz, bool host_in, bool own_pv, bool unsorted,
                       BatchBufs& B);
-void batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
+hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
                     uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort);
 
 uint64_t tc_ws_words(uint32_t X);
-void tc_launch(const TcView& view, const TcOut& out, const TcParams& prm, uint64_t* ws, int num_cus,
-               hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end);
+// Enqueues binning and the intersection kernels; returns the launch status.
+hipError_t tc_launch(const TcView& view, const TcOut& out, const TcParams& prm, uint64_t* ws, int num_cus,
+                     hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end);
 
 }  // namespace kt

@@ -217,7 +217,8 @@ static int launch(ServeState* S, ServeSlot& sl, bool sort) {
         return KT_EHIP;
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, (uint64_t)S->num_cus * 4));
     k_serve_gather<<<grid, 256, 0, st>>>(G, d_desc, chunks, d_off, d_heads, N, nnz);
-    batch_pipeline(st, S->num_cus, S->prm, B, G, N, nnz, B.pv, sl.h_totals, sl.h_small, sort);
+    if (batch_pipeline(st, S->num_cus, S->prm, B, G, N, nnz, B.pv, sl.h_totals, sl.h_small, sort) != hipSuccess)
+        return KT_EHIP;
     for (uint32_t g = 0; g < G; ++g) {
         ServeReq* r = sl.reqs[g];
         if (r->pv_out && r->n &&

@@ -37,25 +37,21 @@ constexpr int SH_ITEMS = 16;
 constexpr int SH_TILE = SH_THREADS * SH_ITEMS;
 
 // Global id -> combined index (0xffffffff: unknown to this rank).  Ghosts are
-// looked up in a dense table over all global ids when it fits (one gather),
-// else by binary search in the sorted ghost ids.
+// ranked in a bitmap over all global ids: rs[w] = {ghost bits of ids 32w ..
+// 32w+31, ghosts below 32w}, so a lookup is one 8-byte gather and a popcount
+// (n_all / 4 bytes in all, a table the caches hold far better than one index
+// per id).
 struct ShardMap {
     uint32_t from, to, nl, ng;
     const uint32_t* ghost_ids;
-    const uint32_t* dense;  // n_all entries, or nullptr
+    const uint2* rs;
     uint32_t n_all;
     __device__ __forceinline__ uint32_t map(uint32_t u) const {
         if (u >= from && u < to) return u - from;
-        if (dense) return u < n_all ? dense[u] : 0xffffffffu;
-        uint32_t lo = 0, hi = ng;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (ghost_ids[mid] < u)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        return (lo < ng && ghost_ids[lo] == u) ? nl + lo : 0xffffffffu;
+        if (u >= n_all) return 0xffffffffu;
+        const uint2 e = rs[u >> 5];
+        const uint32_t bit = 1u << (u & 31);
+        return (e.x & bit) ? nl + e.y + (uint32_t)__popc(e.x & (bit - 1)) : 0xffffffffu;
     }
     __device__ __forceinline__ uint32_t gid(uint32_t c) const { return c < nl ? from + c : ghost_ids[c - nl]; }
 };
@@ -84,16 +80,27 @@ __global__ void k_shard_prepare(ShardMap M, uint64_t nnz, const uint64_t* __rest
     }
 }
 
-// dense[ghost_ids[g]] = nl + g (the table is pre-filled with 0xffffffff)
-__global__ void k_dense_ghosts(uint32_t nl, uint32_t ng, const uint32_t* __restrict__ ghost_ids, uint32_t n_all,
-                               uint32_t* __restrict__ dense, uint32_t* __restrict__ err) {
+// Ghost bits of the rank-select table (zeroed before): word w of the table
+// is rs32[2w], its prefix count rs32[2w + 1].
+__global__ void k_ghost_bits(uint32_t ng, const uint32_t* __restrict__ ghost_ids, uint32_t n_all,
+                             uint32_t* __restrict__ rs32, uint32_t* __restrict__ err) {
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
         const uint32_t id = ghost_ids[g];
         if (id < n_all)
-            dense[id] = nl + g;
+            atomicOr(&rs32[2 * (id >> 5)], 1u << (id & 31));
         else
             atomicOr(err, KT_ERR_GHOST);
     }
+}
+
+__global__ void k_rs_count(uint64_t words, const uint2* __restrict__ rs, uint64_t* __restrict__ cnt) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x)
+        cnt[w] = __popc(rs[w].x);
+}
+
+__global__ void k_rs_prefix(uint64_t words, const uint64_t* __restrict__ pre, uint2* __restrict__ rs) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x)
+        rs[w].y = (uint32_t)pre[w];
 }
 
 __device__ __forceinline__ bool c_less(const ShardMap& M, const uint32_t* deg, uint32_t a, uint32_t b) {
@@ -211,10 +218,43 @@ __global__ void k_cut_begin(uint32_t nl, const uint64_t* __restrict__ coff, cons
 }
 
 // Messages are walked by groups of MSG_G lanes (one message per group; most
-// messages are short), with ballots masked to the group.
+// messages are short), each lane taking MSG_E consecutive entries per step so
+// that a message of up to 64 ids takes one step, its loads and rank-select
+// gathers all in flight together.
 constexpr unsigned MSG_G = 16;
+constexpr unsigned MSG_E = 4;
 
-__device__ __forceinline__ uint64_t group_mask() { return 0xffffull << (lane_id() & ~(MSG_G - 1)); }
+__device__ __forceinline__ uint32_t group_sum(uint32_t x) {
+#pragma unroll
+    for (unsigned d = MSG_G / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, MSG_G);
+    return x;
+}
+
+// Inclusive prefix sum over the lanes of this lane's group (every lane of the
+// group active).
+__device__ __forceinline__ uint32_t group_incl_sum(uint32_t x) {
+    const unsigned gl = lane_id() & (MSG_G - 1);
+#pragma unroll
+    for (unsigned d = 1; d < MSG_G; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, MSG_G);
+        if (gl >= d) x += y;
+    }
+    return x;
+}
+
+// One step of a group over entries [p0, p0 + MSG_G * MSG_E) of a message
+// ending at e: this lane's entries p0 + MSG_E * gl + k, their combined
+// indices (0xffffffff past the end or unknown to the rank).
+struct MsgStep {
+    uint32_t x[MSG_E], c[MSG_E];
+    __device__ __forceinline__ void load(const ShardMap& M, const uint32_t* __restrict__ mheads, uint64_t p,
+                                         uint64_t e) {
+#pragma unroll
+        for (unsigned k = 0; k < MSG_E; ++k) x[k] = (p + k < e) ? mheads[p + k] : 0u;
+#pragma unroll
+        for (unsigned k = 0; k < MSG_E; ++k) c[k] = (p + k < e) ? M.map(x[k]) : 0xffffffffu;
+    }
+};
 
 // Per message: owned / ghost entry counts; sender -> combined index.
 __global__ void k_msg_count(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint32_t* __restrict__ src,
@@ -224,25 +264,29 @@ __global__ void k_msg_count(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / MSG_G;
     const unsigned gl = lane_id() & (MSG_G - 1);
-    const uint64_t gmask = group_mask();
     uint32_t flags = 0;
     for (uint64_t i = t / MSG_G; i < nmsg; i += groups) {
         const uint64_t b = moff[i], e = moff[i + 1];
         const bool bad = e < b || e > mtot;
-        uint32_t own = 0, gh = 0;
+        uint32_t own = 0, gh = 0;  // this lane's share
         if (!bad) {
-            for (uint64_t p0 = b; p0 < e; p0 += MSG_G) {  // uniform over the group
-                const uint64_t p = p0 + gl;
-                uint32_t c = 0xffffffffu;
-                if (p < e) {
-                    const uint32_t x = mheads[p];
-                    c = M.map(x);
-                    if (p + 1 < e && mheads[p + 1] <= x) flags |= KT_ERR_MESSAGE;
+            for (uint64_t p0 = b; p0 < e; p0 += MSG_G * MSG_E) {  // uniform over the group
+                const uint64_t p = p0 + MSG_E * gl;
+                MsgStep s;
+                s.load(M, mheads, p, e);
+                const uint32_t xn = (p + MSG_E < e) ? mheads[p + MSG_E] : 0u;  // the next lane's first entry
+#pragma unroll
+                for (unsigned k = 0; k < MSG_E; ++k) {
+                    if (p + k >= e) continue;
+                    const uint32_t nx = (k + 1 < MSG_E) ? s.x[k + 1] : xn;
+                    if (p + k + 1 < e && nx <= s.x[k]) flags |= KT_ERR_MESSAGE;
+                    own += s.c[k] < M.nl ? 1u : 0u;
+                    gh += (s.c[k] != 0xffffffffu && s.c[k] >= M.nl) ? 1u : 0u;
                 }
-                own += (uint32_t)__popcll(__ballot(c < M.nl) & gmask);
-                gh += (uint32_t)__popcll(__ballot(c != 0xffffffffu && c >= M.nl) & gmask);
             }
         }
+        own = group_sum(own);
+        gh = group_sum(gh);
         if (gl == 0) {
             const uint32_t a = M.map(src[i]);
             if (bad || a == 0xffffffffu || a < M.nl) flags |= KT_ERR_MESSAGE;
@@ -262,23 +306,33 @@ __global__ void k_msg_write(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / MSG_G;
     const unsigned gl = lane_id() & (MSG_G - 1);
-    const uint64_t gmask = group_mask();
     for (uint64_t i = t / MSG_G; i < nmsg; i += groups) {
         const uint64_t b = moff[i], e = moff[i + 1];
         uint64_t po = coff_msg[i], pg = coff_msg[i] + nown[i];
         if (gl == 0) split[i] = pg;
         if (e < b || e > mtot) continue;  // flagged by k_msg_count; nothing was counted for it
-        for (uint64_t p0 = b; p0 < e; p0 += MSG_G) {
-            const uint64_t p = p0 + gl;
-            uint32_t c = 0xffffffffu;
-            if (p < e) c = M.map(mheads[p]);
-            const bool own = c < M.nl;
-            const bool gh = c != 0xffffffffu && c >= M.nl;
-            const uint64_t mo = __ballot(own) & gmask, mg = __ballot(gh) & gmask;
-            if (own) out[po + __popcll(mo & lanemask_lt())] = c;
-            if (gh) out[pg + __popcll(mg & lanemask_lt())] = c;
-            po += __popcll(mo);
-            pg += __popcll(mg);
+        for (uint64_t p0 = b; p0 < e; p0 += MSG_G * MSG_E) {
+            const uint64_t p = p0 + MSG_E * gl;
+            MsgStep s;
+            s.load(M, mheads, p, e);
+            uint32_t o = 0, h = 0;
+#pragma unroll
+            for (unsigned k = 0; k < MSG_E; ++k) {
+                o += s.c[k] < M.nl ? 1u : 0u;
+                h += (s.c[k] != 0xffffffffu && s.c[k] >= M.nl) ? 1u : 0u;
+            }
+            const uint32_t oi = group_incl_sum(o), hi = group_incl_sum(h);
+            const uint32_t ot = __shfl(oi, MSG_G - 1, MSG_G), ht = __shfl(hi, MSG_G - 1, MSG_G);
+            uint64_t qo = po + (oi - o), qg = pg + (hi - h);
+#pragma unroll
+            for (unsigned k = 0; k < MSG_E; ++k) {
+                if (s.c[k] < M.nl)
+                    out[qo++] = s.c[k];
+                else if (s.c[k] != 0xffffffffu)
+                    out[qg++] = s.c[k];
+            }
+            po += ot;
+            pg += ht;
         }
     }
 }
@@ -320,12 +374,11 @@ struct ShardBufs {
     uint64_t* scan_tmp;
     uint64_t* tc_ws;
     uint64_t* pv;
-    uint32_t* dense;
+    uint2* rs;
+    uint64_t* rs_cnt;
     uint32_t* tbnd;
 };
 
-// A dense global-id table is used up to 2^30 ids (4 GiB).
-constexpr uint64_t DENSE_MAX_IDS = 1ull << 30;
 
 uint64_t shard_layout(void* base, const kt_shard_in* in, uint64_t nnz, uint64_t mtot, bool host_in, bool pv_dev,
                       ShardBufs& B) {
@@ -355,11 +408,14 @@ uint64_t shard_layout(void* base, const kt_shard_in* in, uint64_t nnz, uint64_t 
     B.mtail = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 4));
     B.msplit = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 8));
     B.mheads2 = static_cast<uint32_t*>(cv.take((mtot + 4) * 4));
-    B.scan_tmp = static_cast<uint64_t*>(cv.take(scan_tmp_words(std::max<uint64_t>(X, in->n_msg) + 1) * 8));
+    const uint64_t n_all = in->range_bounds[in->num_ranks];
+    const uint64_t rs_words = (n_all + 31) / 32;
+    B.scan_tmp = static_cast<uint64_t*>(
+        cv.take(scan_tmp_words(std::max(std::max<uint64_t>(X, in->n_msg), rs_words) + 1) * 8));
     B.tc_ws = static_cast<uint64_t*>(cv.take(tc_ws_words((uint32_t)std::max<uint64_t>(X, in->n_msg)) * 8));
     B.pv = pv_dev ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(X, 1) * 8)) : nullptr;
-    const uint64_t n_all = in->range_bounds[in->num_ranks];
-    B.dense = n_all <= DENSE_MAX_IDS ? static_cast<uint32_t*>(cv.take(std::max<uint64_t>(n_all, 1) * 4)) : nullptr;
+    B.rs = static_cast<uint2*>(cv.take((rs_words + 1) * 8));
+    B.rs_cnt = static_cast<uint64_t*>(cv.take((rs_words + 1) * 8));
     B.tbnd = static_cast<uint32_t*>(cv.take((2 * ((nnz + SH_TILE - 1) / SH_TILE) + 1) * 4));
     return cv.used;
 }
@@ -443,12 +499,17 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
     uint64_t* nkeys = B.small + 1;
     fill_u64(B.small, 64, 0, st);
     const uint64_t n_all = bd[P];
-    ShardMap M{from, to, nl, ng, B.ghost_ids, B.dense, (uint32_t)std::min<uint64_t>(n_all, 0xffffffffull)};
-    if (B.dense) {
-        fill_u32(B.dense, n_all, 0xffffffffu, st);
+    ShardMap M{from, to, nl, ng, B.ghost_ids, B.rs, (uint32_t)std::min<uint64_t>(n_all, 0xffffffffull)};
+    {
+        const uint64_t rs_words = (n_all + 31) / 32;
+        fill_u64(reinterpret_cast<uint64_t*>(B.rs), rs_words, 0, st);
         if (ng)
-            k_dense_ghosts<<<(unsigned)std::min<uint64_t>((ng + 255) / 256, 8192), 256, 0, st>>>(nl, ng, B.ghost_ids,
-                                                                                               M.n_all, B.dense, err);
+            k_ghost_bits<<<(unsigned)std::min<uint64_t>((ng + 255) / 256, 8192), 256, 0, st>>>(
+                ng, B.ghost_ids, M.n_all, reinterpret_cast<uint32_t*>(B.rs), err);
+        const unsigned gw = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((rs_words + 255) / 256, 16384));
+        k_rs_count<<<gw, 256, 0, st>>>(rs_words, B.rs, B.rs_cnt);
+        scan_u64(B.rs_cnt, B.rs_cnt, rs_words, B.scan_tmp, st);
+        k_rs_prefix<<<gw, 256, 0, st>>>(rs_words, B.rs_cnt, B.rs);
     }
     const int bits = std::max(1, bits_for(X));  // X itself is never a valid index: keys stay below 2^(2 bits)
     const bool cetric = in->algorithm == KT_SHARD_CETRIC;
@@ -515,7 +576,9 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
             v.theads = B.cheads;
         }
         TcOut o{pv, totals};
-        tc_launch(v, o, c->prm, B.tc_ws, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]);
+        rc = c->hip_check(tc_launch(v, o, c->prm, B.tc_ws, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]),
+                          "intersection launch");
+        if (rc) return rc;
         rc = c->hip_check(hipMemcpyAsync(c->host_small + 8, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
         if (rc) return rc;
     }
@@ -539,7 +602,8 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
         v.theads = B.cheads;
         v.tail_id = B.mtail;
         TcOut o{pv, totals + 1};
-        tc_launch(v, o, c->prm, B.tc_ws, c->num_cus, st, nullptr, nullptr);
+        rc = c->hip_check(tc_launch(v, o, c->prm, B.tc_ws, c->num_cus, st, nullptr, nullptr), "intersection launch");
+        if (rc) return rc;
         rc = c->hip_check(hipMemcpyAsync(c->host_small + 12, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
         if (rc) return rc;
     }

@@ -18,8 +18,9 @@
 //     for x -- and flushed with one global atomic per non-zero counter;
 //   * tails are bucketed by ceil(log2 |S(x)|) and taken largest first from
 //     per-bucket work queues (dynamic, LPT-like balance across the grid);
-//   * tails whose S(x) exceeds the LDS table are probed by binary search in
-//     the sorted S(x) in global memory instead.
+//   * tails above the workgroup bin go to 16-wave teams whose table fills a
+//     workgroup's LDS (16384 slots, |S| <= 8192); only larger S(x) are probed
+//     by binary search in the sorted S(x) in global memory.
 //
 // Counts are exact integers; the hash only changes where a set member is
 // stored, never which members match.  T lists are read in aligned 16-byte
@@ -40,19 +41,27 @@ constexpr uint32_t MIN_LOGS = 6;
 // Work-queue buckets.  By degree: bucket b < 12 holds tails with |S| in
 // (2^(b-1), 2^b].  By vertex: buckets 0..7 hold the workgroup tails and 8..15
 // the wave tails of the eight contiguous eighths of the tail range, one per
-// XCD, each in vertex order.  Bucket 16 holds the tails too large for LDS.
-constexpr int NBUCKET = 17;
+// XCD, each in vertex order.  Bucket 17 holds the tails above the workgroup
+// bin whose table still fits a workgroup's whole LDS (k_tc_big), bucket 16
+// those beyond (binary search, k_tc_huge).
+constexpr int NBUCKET = 18;
 constexpr int BUCKET_HUGE = 16;
+constexpr int BUCKET_BIG = 17;
+// k_tc_big: one 1024-thread workgroup per CU with a 16384-slot table (128 KB
+// of LDS for keys and counters), for |S| <= 8192
+constexpr int BIG_THREADS = 1024;
+constexpr int BIG_WAVES = BIG_THREADS / KT_WAVE;
+constexpr uint32_t BIG_SLOTS = 16384;
 constexpr int NXCD = 8;  // MI355X: workgroup i runs on XCD i % 8
 constexpr uint32_t WAVE_GRAB = 8;  // wave-phase tails taken per queue access
 
 // work-queue header (u32 words at the start of the workspace); every counter
 // that waves update concurrently sits on its own 128-byte line so that the
 // queues of different buckets land on different L2 channels:
-//   [0, 17)               tails per bucket (written by binning, then read-only)
+//   [0, 18)               tails per bucket (written by binning, then read-only)
 //   Q_PULL + 32 * b       pull cursor of bucket b
 //   Q_FILL + 32 * b       fill cursor of bucket b (binning pass 2)
-enum : int { Q_CNT = 0, Q_PULL = 32, Q_FILL = 32 * 18, Q_STRIDE = 32, Q_WORDS = 32 * 36 };
+enum : int { Q_CNT = 0, Q_PULL = 32, Q_FILL = 32 * (NBUCKET + 1), Q_STRIDE = 32, Q_WORDS = 32 * (2 * NBUCKET + 2) };
 // stats words after the queue header (the reference's work counters)
 enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_WORDS = 4 };
 
@@ -118,7 +127,7 @@ __device__ __forceinline__ int tail_bucket(const View<PLAIN>& V, const TcParams&
     s = se - sb;
     r = re - rb;
     if (s == 0 || r == 0 || (PLAIN && s < 2)) return -1;
-    if (s > prm.group_max_deg) return BUCKET_HUGE;
+    if (s > prm.group_max_deg) return s > BIG_SLOTS / 2 ? BUCKET_HUGE : BUCKET_BIG;
     const int b = (int)ceil_log2(s);  // 0..11
     if (prm.order == TC_BY_VERTEX) {  // two classes per XCD, each in vertex order
         const int xcd = (int)(((uint64_t)x * NXCD) / V.v.X);
@@ -220,6 +229,14 @@ struct Lds {
     uint32_t cnt[GROUP_SLOTS];
     PackLds pk[TC_WAVES];
     uint64_t red[TC_WAVES];
+    uint32_t next_u;
+    uint32_t bcast;
+};
+
+// Team control words of k_tc_big (its table is dynamic LDS).
+struct LdsBig {
+    PackLds pk[BIG_WAVES];
+    uint64_t red[BIG_WAVES];
     uint32_t next_u;
     uint32_t bcast;
 };
@@ -544,8 +561,8 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
 }
 
 // Process tail x with a team of NW waves (NW == 1: this wave alone).
-template <int NW, bool HASH, bool PLAIN, int GL>
-__device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, Lds& L, uint32_t* keys,
+template <int NW, bool HASH, bool PLAIN, int GL, class LdsT>
+__device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, LdsT& L, uint32_t* keys,
                                              uint32_t* cnt, uint32_t cap_slots, uint32_t chunk, const TcOut& out,
                                              uint64_t& my_total, uint64_t& st_wedge, uint64_t& st_isz) {
     const unsigned lane = lane_id();
@@ -755,6 +772,38 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
     }
 }
 
+// Tails above the workgroup bin whose hash table fits one workgroup's LDS
+// (|S| <= 8192): a 16-wave team per tail with a 16384-slot table in dynamic
+// LDS, one workgroup per CU.
+template <bool PLAIN>
+__global__ void __launch_bounds__(BIG_THREADS) k_tc_big(View<PLAIN> V, TcOut out, TcParams prm,
+                                                        uint32_t* __restrict__ q, const uint32_t* __restrict__ list,
+                                                        uint64_t* __restrict__ stats) {
+    __shared__ LdsBig L;
+    extern __shared__ uint32_t big_table[];  // keys[BIG_SLOTS], then cnt[BIG_SLOTS]
+    uint64_t my_total = 0, st_wedge = 0, st_isz = 0;
+    const uint32_t cnt = q[Q_CNT + BUCKET_BIG];
+    const uint32_t base = bucket_base(q, BUCKET_BIG);
+    uint32_t* pull = &q[Q_PULL + Q_STRIDE * BUCKET_BIG];
+    while (true) {
+        if (threadIdx.x == 0)
+            L.bcast = (__hip_atomic_load(pull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt)
+                          ? cnt
+                          : atomicAdd(pull, 1u);
+        __syncthreads();
+        const uint32_t i = L.bcast;
+        __syncthreads();
+        if (i >= cnt) break;
+        team_process<BIG_WAVES, true, PLAIN, KT_WAVE>(V, list[base + i], L, big_table, big_table + BIG_SLOTS,
+                                                      BIG_SLOTS, prm.team_chunk, out, my_total, st_wedge, st_isz);
+    }
+    if (lane_id() == 0) {
+        if (my_total) atomic_add_u64(&out.totals[0], my_total);
+        if (st_wedge) atomic_add_u64(&stats[ST_WEDGE], st_wedge);
+        if (st_isz) atomic_add_u64(&stats[ST_ISZ], st_isz);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -766,8 +815,9 @@ uint64_t tc_ws_words(uint32_t X) {
 uint64_t* tc_stats_ptr(uint64_t* ws) { return ws + Q_WORDS / 2; }
 
 template <bool PLAIN>
-static void launch_impl(const TcView& view, const TcOut& out, const TcParams& prm, uint64_t* ws, int num_cus,
-                        hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
+static hipError_t launch_impl(const TcView& view, const TcOut& out, const TcParams& prm, uint64_t* ws, int num_cus,
+                              hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
+    (void)hipGetLastError();  // a status left by an earlier call is not this launch's
     View<PLAIN> V;
     V.v = view;
     uint32_t* q = reinterpret_cast<uint32_t*>(ws);
@@ -787,9 +837,16 @@ static void launch_impl(const TcView& view, const TcOut& out, const TcParams& pr
         k_tc_bin_fill<PLAIN><<<g, 256, 0, st>>>(V, p, q, list);
     }
     unsigned grid = (unsigned)num_cus * (p.grid_per_cu ? p.grid_per_cu : 4);
-    // huge tails are rare: a small grid; it exits at once when there are none
+    // huge and big tails are rare: one workgroup per CU; they exit at once
+    // when there are none
     k_tc_huge<PLAIN><<<(unsigned)num_cus, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
-    if (ev_begin) hipEventRecord(ev_begin, st);
+    constexpr uint32_t big_lds = 2 * BIG_SLOTS * sizeof(uint32_t);
+    static const hipError_t big_ok =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tc_big<PLAIN>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)big_lds);
+    if (big_ok != hipSuccess) return big_ok;
+    k_tc_big<PLAIN><<<(unsigned)num_cus, BIG_THREADS, big_lds, st>>>(V, out, p, q, list, stats);
+    if (ev_begin) (void)hipEventRecord(ev_begin, st);
     // lanes per list: the whole wave (streamed) or groups of 32, 16 or 8
     switch (p.strategy) {
         case TC_GROUPED32: k_tc_main<PLAIN, 32><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
@@ -797,17 +854,16 @@ static void launch_impl(const TcView& view, const TcOut& out, const TcParams& pr
         case TC_GROUPED8: k_tc_main<PLAIN, 8><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
         default: k_tc_main<PLAIN, KT_WAVE><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
     }
-    if (ev_end) hipEventRecord(ev_end, st);
+    if (ev_end) (void)hipEventRecord(ev_end, st);
+    return hipGetLastError();  // a launch that did not start (e.g. LDS beyond the device's limit)
 }
 
-void tc_launch(const TcView& view, const TcOut& out, const TcParams& prm, uint64_t* ws, int num_cus,
-               hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
+hipError_t tc_launch(const TcView& view, const TcOut& out, const TcParams& prm, uint64_t* ws, int num_cus,
+                     hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) {
     const bool plain = view.s_beg == nullptr && view.r_beg == nullptr && view.rheads == nullptr &&
                        view.t_beg == nullptr && view.theads == nullptr && view.tail_id == nullptr;
-    if (plain)
-        launch_impl<true>(view, out, prm, ws, num_cus, st, ev_begin, ev_end);
-    else
-        launch_impl<false>(view, out, prm, ws, num_cus, st, ev_begin, ev_end);
+    return plain ? launch_impl<true>(view, out, prm, ws, num_cus, st, ev_begin, ev_end)
+                 : launch_impl<false>(view, out, prm, ws, num_cus, st, ev_begin, ev_end);
 }
 
 }  // namespace kt
