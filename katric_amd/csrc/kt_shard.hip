@@ -602,7 +602,12 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
         v.theads = B.cheads;
         v.tail_id = B.mtail;
         TcOut o{pv, totals + 1};
-        rc = c->hip_check(tc_launch(v, o, c->prm, B.tc_ws, c->num_cus, st, nullptr, nullptr), "intersection launch");
+        // message tails have short owned parts: a whole workgroup on one of
+        // them idles, so tails up to the largest wave table stay with one
+        // wave (measured on RMAT-22 and RMAT-26 / 16 ranks)
+        TcParams pm = c->prm;
+        pm.wave_max_deg = std::max<uint32_t>(pm.wave_max_deg, 256u);
+        rc = c->hip_check(tc_launch(v, o, pm, B.tc_ws, c->num_cus, st, nullptr, nullptr), "intersection launch");
         if (rc) return rc;
         rc = c->hip_check(hipMemcpyAsync(c->host_small + 12, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
         if (rc) return rc;

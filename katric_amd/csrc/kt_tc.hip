@@ -653,8 +653,11 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     team_sync<NW>();  // table reuse by the next tail
 }
 
+// Streamed walks fit 64 VGPRs (8 waves per SIMD, four workgroups per CU);
+// grouped walks keep more list state per lane and get 80 VGPRs (6 waves per
+// SIMD), which keeps their step loop free of register spills.
 template <bool PLAIN, int GL>
-__global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) k_tc_main(View<PLAIN> V, TcOut out, TcParams prm,
+__global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu(GL < KT_WAVE ? 6 : 8, 8))) k_tc_main(View<PLAIN> V, TcOut out, TcParams prm,
                                                          uint32_t* __restrict__ q,
                                                          const uint32_t* __restrict__ list,
                                                          uint64_t* __restrict__ stats) {
