@@ -42,8 +42,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # this workload under its autotuned knobs (FETCH_SIZE, then WRITE_SIZE, KiB
 # per dispatch).  A plain 1 GiB device copy reads FETCH_SIZE = 524,308 KiB and
 # WRITE_SIZE = 1,048,576 KiB on MI355X, so FETCH_SIZE is scaled by 2.000 and
-# WRITE_SIZE by 1.000: 57,644,887 KiB x 2.000 + 2,819,597 KiB = 120.94 GB.
-MEASURED_TRAFFIC = {("plain", 22, 16, 1): 120.94e9}
+# WRITE_SIZE by 1.000: 57,571,680 KiB x 2.000 + 2,829,838 KiB = 120.80 GB.
+MEASURED_TRAFFIC = {("plain", 22, 16, 1): 120.80e9}
 
 
 
