@@ -221,8 +221,8 @@ __global__ void k_cut_begin(uint32_t nl, const uint64_t* __restrict__ coff, cons
 // messages are short), each lane taking MSG_E consecutive entries per step so
 // that a message of up to 64 ids takes one step, its loads and rank-select
 // gathers all in flight together.
-constexpr unsigned MSG_G = 16;
-constexpr unsigned MSG_E = 4;
+constexpr unsigned MSG_G = 8;
+constexpr unsigned MSG_E = 8;
 
 __device__ __forceinline__ uint32_t group_sum(uint32_t x) {
 #pragma unroll
