@@ -96,6 +96,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_info():
+    """The host the CPU baseline ran on: CPU model and logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -203,7 +217,7 @@ def run_shard(args):
             assert int(opv.sum(dtype=np.uint64)) == int(pv.sum(dtype=np.uint64)), "CPU counts outside the shard"
             result["config"]["checked"] = "local, type-3 and per-vertex counts equal the CPU rank simulation"
         result["cpu_baseline"] = {
-            "value": units / cpu_s, "unit": "edges/s", "cores": thr, "kind": "port",
+            "value": units / cpu_s, "unit": "edges/s", "cores": thr, "kind": "port", **host_info(),
             "sample": f"full rank {args.rank} simulation (whole-graph degree orientation, ghost lists, message "
                       f"lists, local + contracted phase, {thr} OpenMP threads), {cpu_s:.2f} s",
         }
@@ -604,6 +618,7 @@ def run_plain(args):
             "unit": "edges/s",
             "cores": thr,
             "kind": "port",
+            **host_info(),
             "sample": f"full RMAT-{args.scale} workload, minimum of {len(runs)} runs (orient + sort + "
                       f"merge-intersection local phase, {thr} OpenMP threads), {cpu_s:.2f} s",
         }
@@ -617,7 +632,7 @@ def run_plain(args):
         if not args.no_check:
             assert sl + s3 == ctx.count(gs, per_vertex=False)[0], "single-rank CPU count != GPU count on the sample"
         result["cpu_baseline_single_rank"] = {
-            "value": gs.m / one_s, "unit": "edges/s", "cores": 1, "kind": "port",
+            "value": gs.m / one_s, "unit": "edges/s", "cores": 1, "kind": "port", **host_info(),
             "sample": f"RMAT-{ss} ef{args.edge_factor} seed={args.seed} ({gs.m} edges), single-rank katric "
                       f"(CeTriC preset, P = 1), one thread, {one_s:.2f} s",
         }
