@@ -41,6 +41,7 @@ extern "C" {
 #define KT_ECANCELED (-6) /* serve: request cancelled */
 #define KT_ENOTFOUND (-7) /* serve: unknown ticket */
 #define KT_ESTATE (-8)    /* call not valid in the current state */
+#define KT_ERANGE (-9)    /* output buffer too small (the required size is still returned) */
 
 /* flags */
 #define KT_DEVICE_INPUT 0x1u  /* offsets / heads are device pointers */
@@ -128,6 +129,20 @@ int kt_synchronize(kt_ctx* ctx);
  * 888-1066) whose emit callback counts triangles. */
 int kt_count(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
              uint64_t* total_out, uint64_t* per_vertex_out);
+
+/* Triangle listing: every triangle of a symmetric CSR once, as the
+ * reference's emit callback receives it -- Triangle{v, u, w} with v the tail
+ * (lowest in the (degree, id) order), u in N+(v), w in N+(v) and N+(u)
+ * (run_local's emit, counters/cetric_edge_iterator.h:173-298, the call at
+ * :289; run_shmem's emit, shared_memory_cetric.h:130-147).  triangles
+ * receives 3 ids per triangle (host, or device with KT_DEVICE_OUTPUT), in no
+ * particular order (the reference emits concurrently from its workers);
+ * *total_out the number of triangles.  If it exceeds capacity (in
+ * triangles), nothing is written and KT_ERANGE is returned with *total_out
+ * set.  Two device passes: a count with per-tail totals, then the listing
+ * at the scanned per-tail offsets. */
+int kt_list_triangles(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+                      uint32_t* triangles, uint64_t capacity, uint64_t* total_out);
 
 /* Degree orientation + oriented-CSR construction only: out_offsets[n+1],
  * out_heads[offsets[n]/2] receive, for every v, the ascending ids u with

@@ -13,12 +13,17 @@
 //       include/cetric/datastructures/distributed/distributed_graph.h:797-819
 // and adds the rank-shard and batch entry points.
 //
-// Differences, by design: the triangle callback (emit) is not exported --
-// every reference caller only counts (shared_memory_cetric.h:130-147), and
-// 10^9 callbacks cannot cross the device boundary; the strategy knobs of
-// Config are accepted but the device picks its own intersection strategy
-// (counts are identical for every strategy).  Errors raise katric::Error
-// (the reference aborts through KASSERT in debug builds and has no status).
+//   CetricEdgeIterator::run_local(emit, stats, ...) with its Triangle{v, u, w}
+//       callback: counters/cetric_edge_iterator.h:173-298 (emit at :289),
+//       datastructures/graph_definitions.h:187-190 (Triangle)
+//
+// Differences, by design: run_local lists the triangles on the device and
+// then calls emit on the host, one triangle after another (the reference
+// calls it concurrently from its worker threads, in no particular order);
+// the strategy knobs of Config are accepted but the device picks its own
+// intersection strategy (counts are identical for every strategy).  Errors
+// raise katric::Error (the reference aborts through KASSERT in debug builds
+// and has no status).
 #pragma once
 
 #include <cstdint>
@@ -163,6 +168,36 @@ inline uint64_t count_triangles(Context& ctx, const DistributedGraph& G, std::ve
     return total;
 }
 
+// A triangle as the reference's emit callback receives it: x = v (the tail,
+// lowest in the (degree, id) order), y = u in N+(v), z = w in N+(v) and N+(u).
+struct Triangle {
+    uint32_t x, y, z;
+};
+
+// Every triangle of G once (kt_list_triangles): one counting pass sizes the
+// buffer, the listing pass fills it.
+inline std::vector<Triangle> list_triangles(Context& ctx, const DistributedGraph& G) {
+    const uint32_t n = (uint32_t)G.local_node_count();
+    uint64_t total = 0;
+    int rc = kt_list_triangles(ctx.get(), n, G.first_out().data(), G.head().data(), 0, nullptr, 0, &total);
+    if (rc != KT_OK && rc != KT_ERANGE) ctx.check(rc, "kt_list_triangles");
+    std::vector<Triangle> out(total);
+    if (total)
+        ctx.check(kt_list_triangles(ctx.get(), n, G.first_out().data(), G.head().data(), 0,
+                                    reinterpret_cast<uint32_t*>(out.data()), total, &total),
+                  "kt_list_triangles");
+    return out;
+}
+
+// CetricEdgeIterator::run_local-shaped entry: emit(Triangle) for every
+// triangle of G, then the statistics of the device run.
+template <class Emit>
+void run_local(Context& ctx, const DistributedGraph& G, Emit&& emit, profiling::Statistics& stats) {
+    const std::vector<Triangle> tri = list_triangles(ctx, G);
+    for (const Triangle& t : tri) emit(t);
+    detail::fill_stats(ctx, stats, tri.size());
+}
+
 // One rank's share of CeTriC ("cetric-x") or DiTriC ("patric"): the emits of
 // its local phase and of its contracted / distributed phase (run_cetric_new,
 // counters/cetric.h:665-1002; run_patric, :160-316).
@@ -178,6 +213,22 @@ inline RankCounts count_rank(Context& ctx, const kt_shard_in& in, bool per_verte
     if (per_vertex) r.per_vertex.assign(nl + in.n_ghost, 0);
     kt_shard_out out{0, 0, per_vertex ? r.per_vertex.data() : nullptr};
     ctx.check(kt_count_shard(ctx.get(), &in, &out), "kt_count_shard");
+    r.local_triangles = out.local_triangles;
+    r.type3_triangles = out.type3_triangles;
+    return r;
+}
+
+// Rank `rank` of `num_ranks` contiguous ranges of G, its inputs built on the
+// device from G (kt_count_shard_graph); per_vertex (nullable) receives n
+// counts indexed by global id.
+inline RankCounts count_rank(Context& ctx, const DistributedGraph& G, uint32_t num_ranks, uint32_t rank,
+                             uint32_t algorithm, const std::vector<uint64_t>& range_bounds, bool per_vertex) {
+    RankCounts r;
+    if (per_vertex) r.per_vertex.assign(G.local_node_count(), 0);
+    kt_shard_out out{0, 0, per_vertex ? r.per_vertex.data() : nullptr};
+    ctx.check(kt_count_shard_graph(ctx.get(), (uint32_t)G.local_node_count(), G.first_out().data(), G.head().data(),
+                                   0, algorithm, num_ranks, range_bounds.data(), rank, &out),
+              "kt_count_shard_graph");
     r.local_triangles = out.local_triangles;
     r.type3_triangles = out.type3_triangles;
     return r;

@@ -59,6 +59,7 @@ KT_ENOTREADY = -5
 KT_ECANCELED = -6
 KT_ENOTFOUND = -7
 KT_ESTATE = -8
+KT_ERANGE = -9
 
 
 class KatricError(RuntimeError):
@@ -329,6 +330,23 @@ class Context:
         rc = lib().kt_count(self._h, n, off_p, hd_p, flags, C.byref(total), pv_p)
         self.check(rc, "kt_count")
         return int(total.value)
+
+    def list_triangles(self, g: Graph, capacity: Optional[int] = None) -> np.ndarray:
+        """Every triangle of a host CSR once, as a (T, 3) uint32 array of
+        (v, u, w): v the tail (lowest in the (degree, id) order), u in N+(v),
+        w in N+(v) and N+(u) -- the triples the reference's emit callback
+        receives.  Without `capacity` a first call sizes the buffer."""
+        total = C.c_uint64(0)
+        if capacity is None:
+            rc = lib().kt_list_triangles(self._h, g.n, _ptr(g.offsets), _ptr(g.heads), 0, None, 0, C.byref(total))
+            if rc not in (0, KT_ERANGE):
+                self.check(rc, "kt_list_triangles")
+            capacity = int(total.value)
+        out = np.zeros((max(capacity, 1), 3), dtype=np.uint32)
+        rc = lib().kt_list_triangles(self._h, g.n, _ptr(g.offsets), _ptr(g.heads), 0, out.ctypes.data, capacity,
+                                     C.byref(total))
+        self.check(rc, "kt_list_triangles")
+        return out[: int(total.value)]
 
     def orient(self, g: Graph) -> Graph:
         """Oriented (out-only) CSR under the (degree, id) order, ascending lists."""

@@ -74,6 +74,7 @@ SIGNATURES = {
     "kt_buffer_download": (i32, [vp, vp, vp, u64]),
     "kt_synchronize": (i32, [vp]),
     "kt_count": (i32, [vp, u32, vp, vp, u32, p_u64, vp]),
+    "kt_list_triangles": (i32, [vp, u32, vp, vp, u32, vp, u64, p_u64]),
     "kt_orient": (i32, [vp, u32, vp, vp, u32, vp, vp]),
     # katric_graphio.h
     "kt_host_graph_free": (None, [C.POINTER(KtHostGraph)]),

@@ -91,6 +91,45 @@ int kt_ctx::begin_call(bool counting) {
     return KT_OK;
 }
 
+int kt_ctx::reserve_aux(int i, uint64_t bytes) {
+    bytes = (std::max<uint64_t>(bytes, 256) + 255) & ~255ull;
+    if (bytes <= aux_bytes[i]) return KT_OK;
+    uint64_t want = std::max<uint64_t>(bytes, aux_bytes[i] + aux_bytes[i] / 4);
+    const uint64_t others = arena_bytes + pinned_other - aux_bytes[i];
+    if (budget && others + want > budget) want = bytes;
+    if (budget && others + want > budget)
+        return fail(KT_ENOMEM, "%llu bytes of working memory exceed the HBM budget of %llu bytes",
+                    (unsigned long long)want, (unsigned long long)budget);
+    if (aux[i]) {
+        int rc = hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (rc) return rc;
+        (void)hipFree(aux[i]);
+        pinned_other -= aux_bytes[i];
+        aux[i] = nullptr;
+        aux_bytes[i] = 0;
+    }
+    hipError_t e = hipMalloc(&aux[i], want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        aux[i] = nullptr;
+        return fail(KT_ENOMEM, "hipMalloc(%llu) failed", (unsigned long long)want);
+    }
+    aux_bytes[i] = want;
+    pinned_other += want;
+    return KT_OK;
+}
+
+void kt_ctx::release_aux() {
+    for (int i = 0; i < 2; ++i) {
+        if (aux[i]) {
+            (void)hipFree(aux[i]);
+            pinned_other -= aux_bytes[i];
+        }
+        aux[i] = nullptr;
+        aux_bytes[i] = 0;
+    }
+}
+
 int kt_ctx::read_u64(const uint64_t* dev, uint64_t* out, const char* what) {
     int rc = hip_check(hipMemcpyAsync(host_small + 63, dev, 8, hipMemcpyDeviceToHost, stream), what);
     if (!rc) rc = hip_check(hipStreamSynchronize(stream), what);
@@ -189,6 +228,7 @@ extern "C" const char* kt_strerror(int s) {
         case KT_ECANCELED: return "request cancelled";
         case KT_ENOTFOUND: return "unknown request";
         case KT_ESTATE: return "call not valid in the current state";
+        case KT_ERANGE: return "output buffer too small";
         default: return "unknown status";
     }
 }
@@ -466,6 +506,95 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     return KT_OK;
 }
 
+extern "C" int kt_list_triangles(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads,
+                                 uint32_t flags, uint32_t* triangles, uint64_t capacity, uint64_t* total_out) {
+    if (!c) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(true);
+    if (rc) return rc;
+    if (!total_out || !offsets || (n > 0 && !heads && !(flags & KT_DEVICE_INPUT)))
+        return c->fail(KT_EINVAL, "null argument");
+    *total_out = 0;
+    if (n == 0) return KT_OK;
+    uint64_t nnz = 0;
+    rc = read_nnz(c, n, offsets, flags, nnz);
+    if (rc) return rc;
+    const bool host_in = !(flags & KT_DEVICE_INPUT);
+    const bool dev_out = (flags & KT_DEVICE_OUTPUT) != 0;
+    PlainBufs B;
+    bool unsorted;
+    rc = run_orientation(c, n, nnz, offsets, heads, host_in, false, B, unsorted);
+    if (rc) return rc;
+    // per-tail triangle counts, scanned in place into output offsets
+    Carver cv{nullptr, 0};
+    cv.take(((uint64_t)n + 1) * 8);
+    cv.take(scan_tmp_words((uint64_t)n + 1) * 8);
+    rc = c->reserve_aux(0, cv.used);
+    if (rc) return rc;
+    cv = Carver{c->aux[0], 0};
+    uint64_t* tcnt = static_cast<uint64_t*>(cv.take(((uint64_t)n + 1) * 8));
+    uint64_t* stmp = static_cast<uint64_t*>(cv.take(scan_tmp_words((uint64_t)n + 1) * 8));
+    hipStream_t st = c->stream;
+    (void)hipEventRecord(c->ev[EV_PREP], st);
+    fill_u64(tcnt, (uint64_t)n + 1, 0, st);
+    uint64_t* totals = B.small + 1;
+    TcView view{};
+    view.X = n;
+    view.s_off = B.ooff;
+    view.sheads = B.oheads;
+    TcOut cnt_out{nullptr, totals};
+    cnt_out.tail_count = tcnt;
+    rc = c->hip_check(tc_launch(view, cnt_out, c->prm, B.ws_tc, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]),
+                      "intersection launch");
+    if (rc) return rc;
+    scan_u64(tcnt, tcnt, n, stmp, st);
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tc_stats_ptr(B.ws_tc), 16, hipMemcpyDeviceToHost, st),
+                      "D2H stats");
+    if (rc) return rc;
+    uint64_t total = 0;
+    rc = c->read_u64(tcnt + n, &total, "D2H total");
+    if (rc) return rc;
+    *total_out = total;
+    c->stats.local_triangles = total;
+    c->stats.wedge_checks = c->host_small[2];
+    c->stats.intersection_size_local = c->host_small[3];
+    if (total > capacity)
+        return c->fail(KT_ERANGE, "%llu triangles do not fit a buffer of %llu", (unsigned long long)total,
+                       (unsigned long long)capacity);
+    if (total && !triangles) return c->fail(KT_EINVAL, "null triangle buffer");
+    if (total) {
+        uint32_t* dtri = triangles;
+        if (!dev_out) {
+            rc = c->reserve_aux(1, total * 12);
+            if (rc) return rc;
+            dtri = static_cast<uint32_t*>(c->aux[1]);
+        }
+        TcOut list_out{nullptr, totals};
+        list_out.tail_off = tcnt;
+        list_out.tri = dtri;
+        rc = c->hip_check(tc_launch(view, list_out, c->prm, B.ws_tc, c->num_cus, st, nullptr, nullptr),
+                          "listing launch");
+        if (rc) return rc;
+        (void)hipEventRecord(c->ev[EV_COUNT], st);
+        if (!dev_out) {
+            rc = c->hip_check(hipMemcpyAsync(triangles, dtri, total * 12, hipMemcpyDeviceToHost, st), "D2H triangles");
+            if (rc) return rc;
+        }
+    } else {
+        (void)hipEventRecord(c->ev[EV_COUNT], st);
+    }
+    (void)hipEventRecord(c->ev[EV_END], st);
+    rc = c->hip_check(hipStreamSynchronize(st), "listing");
+    if (rc) return rc;
+    c->timing.h2d_ms = c->elapsed(EV_START, EV_H2D);
+    c->timing.prep_ms = unsorted ? 0.f : c->elapsed(EV_H2D, EV_PREP);
+    c->timing.count_ms = c->elapsed(EV_PREP, EV_COUNT);
+    c->timing.intersect_ms = c->elapsed(EV_TC0, EV_TC1);
+    c->timing.d2h_ms = c->elapsed(EV_COUNT, EV_END);
+    c->timing.total_ms = unsorted ? 0.f : c->elapsed(EV_START, EV_END);
+    return KT_OK;
+}
+
 extern "C" int kt_orient(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
                          uint64_t* out_offsets, uint32_t* out_heads) {
     if (!c) return KT_EINVAL;
@@ -588,7 +717,7 @@ extern "C" int kt_synchronize(kt_ctx* c) {
 void kt_ctx::destroy_extensions() {
     kt_serve_release(this);
     kt_batch_release(this);
-    kt_shard_graph_release(this);
+    release_aux();
     for (auto& kv : buffers) (void)hipFree(kv.first);
     buffers.clear();
 }

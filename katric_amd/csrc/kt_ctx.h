@@ -53,16 +53,20 @@ struct kt_ctx {
     kt::ServeState* serve = nullptr;
     kt::BatchState* batch = nullptr;
     std::unordered_map<void*, uint64_t> buffers;  // kt_buffer_alloc allocations
-    // grow-only device buffers of the device-side shard producer: [0] sized
-    // by the graph and the rank's slice, [1] by the received message volume
-    void* prod[2] = {nullptr, nullptr};
-    uint64_t prod_bytes[2] = {0, 0};
+    // two grow-only device buffers outside the arena, for calls whose second
+    // stage is sized by a result of the first (the device-side shard
+    // producer, triangle listing); their contents last for one call
+    void* aux[2] = {nullptr, nullptr};
+    uint64_t aux_bytes[2] = {0, 0};
 
     int fail(int status, const char* fmt, ...);
     int hip_check(hipError_t e, const char* what);
     int reserve(uint64_t bytes);
     int begin_call(bool counting);
     float elapsed(int a, int b);
+    // Grows aux[i] to at least `bytes` (its contents are dropped).
+    int reserve_aux(int i, uint64_t bytes);
+    void release_aux();
     // One device word, read in order after the work queued on `stream`.
     int read_u64(const uint64_t* dev, uint64_t* out, const char* what);
     void destroy_extensions();
@@ -70,7 +74,6 @@ struct kt_ctx {
 
 void kt_batch_release(kt_ctx* c);
 void kt_serve_release(kt_ctx* c);
-void kt_shard_graph_release(kt_ctx* c);
 
 namespace kt {
 uint64_t* tc_stats_ptr(uint64_t* ws);

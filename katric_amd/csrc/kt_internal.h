@@ -81,6 +81,13 @@ struct TcView {
 struct TcOut {
     uint64_t* per_vertex;       // nullable
     uint64_t* totals;           // per graph (>= 1 entry)
+    // triangle listing (plain views only): a counting launch with tail_count
+    // set records every tail's number of emitted triangles; a listing launch
+    // (tri set) writes tail x's triangles as (x, u, w) id triples from
+    // position tail_off[x] on
+    uint64_t* tail_count = nullptr;
+    const uint64_t* tail_off = nullptr;
+    uint32_t* tri = nullptr;
 };
 
 // Batch pipeline over a disjoint-union CSR (kt_batch.hip), shared with the

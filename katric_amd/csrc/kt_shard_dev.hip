@@ -313,35 +313,6 @@ int bits_of(uint64_t x) {
     return b;
 }
 
-// Grows producer buffer i to at least `bytes` (its contents are dropped).
-int prod_reserve(kt_ctx* c, int i, uint64_t bytes) {
-    bytes = (std::max<uint64_t>(bytes, 256) + 255) & ~255ull;
-    if (bytes <= c->prod_bytes[i]) return KT_OK;
-    uint64_t want = std::max<uint64_t>(bytes, c->prod_bytes[i] + c->prod_bytes[i] / 4);
-    const uint64_t others = c->arena_bytes + c->pinned_other - c->prod_bytes[i];
-    if (c->budget && others + want > c->budget) want = bytes;
-    if (c->budget && others + want > c->budget)
-        return c->fail(KT_ENOMEM, "shard producer needs %llu bytes beyond the HBM budget of %llu bytes",
-                       (unsigned long long)want, (unsigned long long)c->budget);
-    if (c->prod[i]) {
-        int rc = c->hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        if (rc) return rc;
-        (void)hipFree(c->prod[i]);
-        c->pinned_other -= c->prod_bytes[i];
-        c->prod[i] = nullptr;
-        c->prod_bytes[i] = 0;
-    }
-    hipError_t e = hipMalloc(&c->prod[i], want);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        c->prod[i] = nullptr;
-        return c->fail(KT_ENOMEM, "hipMalloc(%llu) for the shard producer failed", (unsigned long long)want);
-    }
-    c->prod_bytes[i] = want;
-    c->pinned_other += want;
-    return KT_OK;
-}
-
 struct ProdBufs {
     uint64_t* small;
     uint64_t* bd;
@@ -440,9 +411,9 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
     const uint64_t ng_max = std::min<uint64_t>((uint64_t)n - nl, nnz_l);
     ProdBufs B;
     const uint64_t need = prod_layout(nullptr, n, nnz, P, nl, ng_max, want_pv, want_pv && !pv_dev, host_in, B);
-    rc = prod_reserve(c, 0, need);
+    rc = c->reserve_aux(0, need);
     if (rc) return rc;
-    prod_layout(c->prod[0], n, nnz, P, nl, ng_max, want_pv, want_pv && !pv_dev, host_in, B);
+    prod_layout(c->aux[0], n, nnz, P, nl, ng_max, want_pv, want_pv && !pv_dev, host_in, B);
     (void)hipEventRecord(c->ev[EV_X0], st);
     const uint64_t* off = offsets;
     const uint32_t* hd = heads;
@@ -516,9 +487,9 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
             cv.take(mtot * 8);
             cv.take(radix_tmp_words(mtot) * 8);
         }
-        rc = prod_reserve(c, 1, cv.used);
+        rc = c->reserve_aux(1, cv.used);
         if (rc) return rc;
-        cv = Carver{c->prod[1], 0};
+        cv = Carver{c->aux[1], 0};
         mheads = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(mtot, 1) * 4));
         k_pd_msgs<true><<<(unsigned)tiles_v, PD_THREADS, 0, st>>>(V, (uint32_t)nmsg, B.voff, B.tbnd, B.sinfo, hd,
                                                                   B.deg, n, nullptr, B.tcnt, mheads, B.moff, err);
@@ -586,17 +557,6 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
 }
 
 }  // namespace
-
-void kt_shard_graph_release(kt_ctx* c) {
-    for (int i = 0; i < 2; ++i) {
-        if (c->prod[i]) {
-            (void)hipFree(c->prod[i]);
-            c->pinned_other -= c->prod_bytes[i];
-        }
-        c->prod[i] = nullptr;
-        c->prod_bytes[i] = 0;
-    }
-}
 
 extern "C" int kt_count_shard_graph(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads,
                                     uint32_t flags, uint32_t algorithm, uint32_t num_ranks,

@@ -229,6 +229,7 @@ struct Lds {
     uint32_t cnt[GROUP_SLOTS];
     PackLds pk[TC_WAVES];
     uint64_t red[TC_WAVES];
+    unsigned long long list_ctr[TC_WAVES];  // listing: triangles of the current tail written so far
     uint32_t next_u;
     uint32_t bcast;
 };
@@ -237,6 +238,7 @@ struct Lds {
 struct LdsBig {
     PackLds pk[BIG_WAVES];
     uint64_t red[BIG_WAVES];
+    unsigned long long list_ctr[BIG_WAVES];
     uint32_t next_u;
     uint32_t bcast;
 };
@@ -362,7 +364,34 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t x, int j) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Triangle listing: the calling lanes (exactly those that hit) write their
+// triangles (x, u, w) of the current tail x from the tail's first output
+// position on, after a team-wide running count kept in LDS.
+// Writes stay inside the tail's own range [base, end) of the output even if
+// this pass found more triangles than the counting pass recorded.
+struct Emitter {
+    uint32_t* tri;
+    uint64_t base, end;
+    unsigned long long* ctr;
+    uint32_t x;
+    __device__ __forceinline__ void operator()(uint32_t u, uint32_t w) const {
+        const uint64_t m = __builtin_amdgcn_ballot_w64(true);
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+        unsigned long long first = 0;
+        if (lane_id() == leader) first = atomicAdd(ctr, (unsigned long long)__popcll(m));
+        first = __shfl(first, (int)leader, 64);
+        const uint64_t pos = base + first + (uint64_t)__popcll(m & lanemask_lt());
+        if (pos < end) {
+            uint32_t* t = tri + 3 * pos;
+            t[0] = x;
+            t[1] = u;
+            t[2] = w;
+        }
+    }
+};
+
 struct ProbeCtx {
+    Emitter em;  // listing launches only
     uint32_t* keys;
     uint32_t* cnt;
     uint32_t logB;   // log2(buckets)
@@ -380,7 +409,7 @@ struct ProbeCtx {
 // same list or the next one -- before probing the current block.  Positions
 // and list bounds are wave-uniform 32-bit values; hits are counted per lane
 // and summed across the wave once per list.  Returns the chunk's hits.
-template <bool HASH, bool PLAIN>
+template <bool HASH, bool PLAIN, bool LIST>
 __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0, uint32_t cnt, const ProbeCtx& P,
                                                uint64_t ns, uint64_t& st_wedge, uint64_t& st_isz) {
     const uint32_t lane = lane_id();
@@ -439,11 +468,18 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
         const bool valid[4] = {0 < lim, 1 < lim, 2 < lim, 3 < lim};
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
         if (HASH) {
-            nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, valid, NoHitAction{});
+            if constexpr (LIST)
+                nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, valid, [&](int e) { P.em(u, w[e]); });
+            else
+                nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, valid, NoHitAction{});
         } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (valid[e]) nh += probe_bsearch(P.s, P.ns, P.pv, w[e]);
+            for (int e = 0; e < 4; ++e) {
+                if (!valid[e]) continue;
+                const uint32_t h = probe_bsearch(P.s, P.ns, P.pv, w[e]);
+                nh += h;
+                if (LIST && h) P.em(u, w[e]);
+            }
         }
         if (switch_u) {
             const uint32_t h = wave_total_dpp(nh);
@@ -561,7 +597,7 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
 }
 
 // Process tail x with a team of NW waves (NW == 1: this wave alone).
-template <int NW, bool HASH, bool PLAIN, int GL, class LdsT>
+template <int NW, bool HASH, bool PLAIN, int GL, bool LIST, class LdsT>
 __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, LdsT& L, uint32_t* keys,
                                              uint32_t* cnt, uint32_t cap_slots, uint32_t chunk, const TcOut& out,
                                              uint64_t& my_total, uint64_t& st_wedge, uint64_t& st_isz) {
@@ -606,6 +642,10 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
         for (uint64_t i = tid; i < ns; i += NW * KT_WAVE) table_insert(keys, P.logB, P.bmask, V.v.sheads[sb + i]);
     }
     if (NW > 1 && tid == 0) L.next_u = NW * chunk;
+    if (LIST) {
+        P.em = Emitter{out.tri, out.tail_off[x], out.tail_off[x + 1], &L.list_ctr[NW > 1 ? 0 : wave], x};
+        if (tid == 0) *P.em.ctr = 0;
+    }
     team_sync<NW>();
 
     uint64_t tail_hits = 0;
@@ -615,7 +655,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
         if (GL < KT_WAVE && HASH)
             tail_hits += scan_chunk_grouped<GL, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz, L.pk[wave]);
         else
-            tail_hits += scan_chunk<HASH, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz);
+            tail_hits += scan_chunk<HASH, PLAIN, LIST>(V, rb + k, c, P, ns, st_wedge, st_isz);
         if (NW > 1) {
             uint32_t nk = 0;
             if (lane == 0) nk = atomicAdd(&L.next_u, chunk);
@@ -643,6 +683,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
             }
         }
     }
+    if (tid == 0 && out.tail_count) out.tail_count[x] = hsum;
     if (tid == 0 && hsum) {
         if (out.per_vertex) atomic_add_u64(&out.per_vertex[V.tail_pv(x)], hsum);
         if (V.v.tail_graph)
@@ -656,7 +697,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
 // Streamed walks fit 64 VGPRs (8 waves per SIMD, four workgroups per CU);
 // grouped walks keep more list state per lane and get 80 VGPRs (6 waves per
 // SIMD), which keeps their step loop free of register spills.
-template <bool PLAIN, int GL>
+template <bool PLAIN, int GL, bool LIST>
 __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu(GL < KT_WAVE ? 6 : 8, 8))) k_tc_main(View<PLAIN> V, TcOut out, TcParams prm,
                                                          uint32_t* __restrict__ q,
                                                          const uint32_t* __restrict__ list,
@@ -688,7 +729,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             const uint32_t i = L.bcast;
             __syncthreads();
             if (i >= cnt) break;
-            team_process<TC_WAVES, true, PLAIN, GL>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.team_chunk,
+            team_process<TC_WAVES, true, PLAIN, GL, LIST>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.team_chunk,
                                                 out, my_total, st_wedge, st_isz);
         }
     }
@@ -716,7 +757,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
                 if (i >= cnt) break;
                 const uint32_t e = min(cnt, i + grab);
                 for (; i < e; ++i)
-                    team_process<1, true, PLAIN, GL>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, KT_WAVE, out,
+                    team_process<1, true, PLAIN, GL, LIST>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, KT_WAVE, out,
                                                  my_total, st_wedge, st_isz);
             }
         }
@@ -746,7 +787,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
 
 // Tails whose S(x) exceeds the LDS table: 8-wave teams, binary-search probes
 // of the sorted S(x) in global memory.
-template <bool PLAIN>
+template <bool PLAIN, bool LIST>
 __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out, TcParams prm,
                                                          uint32_t* __restrict__ q,
                                                          const uint32_t* __restrict__ list,
@@ -765,7 +806,7 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
         const uint32_t i = L.bcast;
         __syncthreads();
         if (i >= cnt) break;
-        team_process<TC_WAVES, false, PLAIN, KT_WAVE>(V, list[base + i], L, nullptr, nullptr, 0, prm.team_chunk, out,
+        team_process<TC_WAVES, false, PLAIN, KT_WAVE, LIST>(V, list[base + i], L, nullptr, nullptr, 0, prm.team_chunk, out,
                                              my_total, st_wedge, st_isz);
     }
     if (lane_id() == 0) {
@@ -778,7 +819,7 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
 // Tails above the workgroup bin whose hash table fits one workgroup's LDS
 // (|S| <= 8192): a 16-wave team per tail with a 16384-slot table in dynamic
 // LDS, one workgroup per CU.
-template <bool PLAIN>
+template <bool PLAIN, bool LIST>
 __global__ void __launch_bounds__(BIG_THREADS) k_tc_big(View<PLAIN> V, TcOut out, TcParams prm,
                                                         uint32_t* __restrict__ q, const uint32_t* __restrict__ list,
                                                         uint64_t* __restrict__ stats) {
@@ -797,7 +838,7 @@ __global__ void __launch_bounds__(BIG_THREADS) k_tc_big(View<PLAIN> V, TcOut out
         const uint32_t i = L.bcast;
         __syncthreads();
         if (i >= cnt) break;
-        team_process<BIG_WAVES, true, PLAIN, KT_WAVE>(V, list[base + i], L, big_table, big_table + BIG_SLOTS,
+        team_process<BIG_WAVES, true, PLAIN, KT_WAVE, LIST>(V, list[base + i], L, big_table, big_table + BIG_SLOTS,
                                                       BIG_SLOTS, prm.team_chunk, out, my_total, st_wedge, st_isz);
     }
     if (lane_id() == 0) {
@@ -840,22 +881,36 @@ static hipError_t launch_impl(const TcView& view, const TcOut& out, const TcPara
         k_tc_bin_fill<PLAIN><<<g, 256, 0, st>>>(V, p, q, list);
     }
     unsigned grid = (unsigned)num_cus * (p.grid_per_cu ? p.grid_per_cu : 4);
+    constexpr uint32_t big_lds = 2 * BIG_SLOTS * sizeof(uint32_t);
+    if constexpr (PLAIN) {
+        if (out.tri) {  // listing: streamed walks only
+            k_tc_huge<PLAIN, true><<<(unsigned)num_cus, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
+            static const hipError_t big_list_ok =
+                hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tc_big<PLAIN, true>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)big_lds);
+            if (big_list_ok != hipSuccess) return big_list_ok;
+            k_tc_big<PLAIN, true><<<(unsigned)num_cus, BIG_THREADS, big_lds, st>>>(V, out, p, q, list, stats);
+            if (ev_begin) (void)hipEventRecord(ev_begin, st);
+            k_tc_main<PLAIN, KT_WAVE, true><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
+            if (ev_end) (void)hipEventRecord(ev_end, st);
+            return hipGetLastError();
+        }
+    }
     // huge and big tails are rare: one workgroup per CU; they exit at once
     // when there are none
-    k_tc_huge<PLAIN><<<(unsigned)num_cus, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
-    constexpr uint32_t big_lds = 2 * BIG_SLOTS * sizeof(uint32_t);
+    k_tc_huge<PLAIN, false><<<(unsigned)num_cus, TC_THREADS, 0, st>>>(V, out, p, q, list, stats);
     static const hipError_t big_ok =
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tc_big<PLAIN>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tc_big<PLAIN, false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)big_lds);
     if (big_ok != hipSuccess) return big_ok;
-    k_tc_big<PLAIN><<<(unsigned)num_cus, BIG_THREADS, big_lds, st>>>(V, out, p, q, list, stats);
+    k_tc_big<PLAIN, false><<<(unsigned)num_cus, BIG_THREADS, big_lds, st>>>(V, out, p, q, list, stats);
     if (ev_begin) (void)hipEventRecord(ev_begin, st);
     // lanes per list: the whole wave (streamed) or groups of 32, 16 or 8
     switch (p.strategy) {
-        case TC_GROUPED32: k_tc_main<PLAIN, 32><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
-        case TC_GROUPED16: k_tc_main<PLAIN, 16><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
-        case TC_GROUPED8: k_tc_main<PLAIN, 8><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
-        default: k_tc_main<PLAIN, KT_WAVE><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+        case TC_GROUPED32: k_tc_main<PLAIN, 32, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+        case TC_GROUPED16: k_tc_main<PLAIN, 16, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+        case TC_GROUPED8: k_tc_main<PLAIN, 8, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+        default: k_tc_main<PLAIN, KT_WAVE, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
     }
     if (ev_end) (void)hipEventRecord(ev_end, st);
     return hipGetLastError();  // a launch that did not start (e.g. LDS beyond the device's limit)

@@ -1,6 +1,7 @@
 // C++ host-layer test: the reference-shaped entry points of
 // include/katric/cetric.hpp against known answers (examples/examples.yaml of
 // the reference) and closed forms.  Needs a GPU; prints one line per check.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -94,6 +95,53 @@ int main(int argc, char** argv) {
             std::printf("shard algo %u over 3 ranks = %llu\n", algo, (unsigned long long)sum);
             CHECK(sum == 45);
         }
+    }
+    {
+        // the same ranks with their inputs built on the device from the graph
+        const auto& G = graphs[2];
+        const uint32_t n = (uint32_t)G.local_node_count();
+        std::vector<uint64_t> bounds = {0, n / 3, 2 * n / 3, n};
+        for (uint32_t algo : {KT_SHARD_CETRIC, KT_SHARD_DITRIC}) {
+            uint64_t sum = 0, pv_sum = 0;
+            for (uint32_t r = 0; r < 3; ++r) {
+                auto rc = katric::count_rank(ctx, G, 3, r, algo, bounds, true);
+                sum += rc.local_triangles + rc.type3_triangles;
+                for (uint64_t x : rc.per_vertex) pv_sum += x;
+            }
+            std::printf("device-produced shards algo %u over 3 ranks = %llu\n", algo, (unsigned long long)sum);
+            CHECK(sum == 45);
+            CHECK(pv_sum == 3 * 45);
+        }
+    }
+    {
+        // run_local-shaped listing: every triangle once, tail first in the
+        // (degree, id) order, all three edges present
+        const auto& G = graphs[2];
+        std::vector<katric::Triangle> seen;
+        katric::profiling::Statistics stats;
+        katric::run_local(ctx, G, [&](const katric::Triangle& t) { seen.push_back(t); }, stats);
+        CHECK(seen.size() == 45 && stats.counted_triangles == 45);
+        auto has_edge = [&](uint32_t a, uint32_t b) {
+            for (uint64_t e = G.first_out()[a]; e < G.first_out()[a + 1]; ++e)
+                if (G.head()[e] == b) return true;
+            return false;
+        };
+        auto before = [&](uint32_t a, uint32_t b) {
+            return G.degree(a) < G.degree(b) || (G.degree(a) == G.degree(b) && a < b);
+        };
+        std::vector<uint64_t> keys;
+        for (const auto& t : seen) {
+            CHECK(has_edge(t.x, t.y) && has_edge(t.x, t.z) && has_edge(t.y, t.z));
+            CHECK(before(t.x, t.y) && before(t.y, t.z));
+            uint32_t a[3] = {t.x, t.y, t.z};
+            std::sort(a, a + 3);
+            keys.push_back(((uint64_t)a[0] << 42) | ((uint64_t)a[1] << 21) | a[2]);
+        }
+        std::sort(keys.begin(), keys.end());
+        CHECK(std::adjacent_find(keys.begin(), keys.end()) == keys.end());
+        auto K = complete(30);
+        CHECK(katric::list_triangles(ctx, K).size() == 30ull * 29 * 28 / 6);
+        std::printf("run_local hiv = %zu triangles listed\n", seen.size());
     }
     {
         bool threw = false;
