@@ -76,6 +76,13 @@ struct TcView {
     const uint32_t* theads;
     const uint32_t* tail_id;    // X (nullptr: identity) -> per-vertex index of the tail
     const uint32_t* tail_graph; // X (nullptr: all graph 0) -> index into per-graph totals
+    // work counters of a message phase: stat_off (X+1, nullable) gives the
+    // length each tail's message adds to intersection_size per receiver (the
+    // whole received message, not only its ghost part S); with
+    // stat_skip_empty, receivers with an empty T(u) add nothing (they are
+    // not in the contracted graph)
+    const uint64_t* stat_off = nullptr;
+    uint32_t stat_skip_empty = 0;
 };
 
 struct TcOut {

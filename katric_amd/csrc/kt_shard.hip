@@ -601,6 +601,11 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
         v.t_end = B.coff + 1;
         v.theads = B.cheads;
         v.tail_id = B.mtail;
+        // counters as intersect_from_message keeps them (cetric_edge_iterator.h:1815-1856):
+        // per local receiver u, |N+(u)| wedge checks and |message| + |N+(u)|
+        // intersected ids; a CeTriC receiver without cut edges was compacted away
+        v.stat_off = B.msg_off;
+        v.stat_skip_empty = cetric ? 1u : 0u;
         TcOut o{pv, totals + 1};
         // message tails have short owned parts: a whole workgroup on one of
         // them idles, so tails up to the largest wave table stay with one

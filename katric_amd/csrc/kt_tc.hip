@@ -104,6 +104,11 @@ struct View {
     __device__ __forceinline__ uint32_t tail_graph(uint32_t x) const {
         return v.tail_graph ? v.tail_graph[x] : 0u;
     }
+    // length tail x charges to intersection_size per receiver (ns = |S(x)|)
+    __device__ __forceinline__ uint64_t stat_len(uint32_t x, uint64_t ns) const {
+        return (!PLAIN && v.stat_off) ? v.stat_off[x + 1] - v.stat_off[x] : ns;
+    }
+    __device__ __forceinline__ bool stat_skip_empty() const { return !PLAIN && v.stat_skip_empty; }
 };
 
 __device__ __forceinline__ uint32_t ceil_log2(uint64_t x) { return x <= 1 ? 0 : 64 - __clzll(x - 1); }
@@ -158,6 +163,20 @@ __global__ void __launch_bounds__(256) k_tc_bin_count(View<PLAIN> V, TcParams pr
             V.t_range(V.r_head(sb), tb, te);
             wedge += te - tb;
             isz += 1 + (te - tb);
+        }
+        // a message with receivers but no ghost entry finds no triangle and
+        // is not binned; its receivers still count (cetric_edge_iterator.h:1850-1856)
+        if (!PLAIN && s == 0 && r != 0) {
+            uint64_t rb, re;
+            V.r_range(x, rb, re);
+            const uint64_t len = V.stat_len(x, 0);
+            for (uint64_t i = rb; i < re; ++i) {
+                uint64_t tb, te;
+                V.t_range(V.r_head(i), tb, te);
+                if (te == tb && V.stat_skip_empty()) continue;
+                wedge += te - tb;
+                isz += len + (te - tb);
+            }
         }
     }
     wedge = wave_sum(wedge);
@@ -411,7 +430,7 @@ struct ProbeCtx {
 // and summed across the wave once per list.  Returns the chunk's hits.
 template <bool HASH, bool PLAIN, bool LIST>
 __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0, uint32_t cnt, const ProbeCtx& P,
-                                               uint64_t ns, uint64_t& st_wedge, uint64_t& st_isz) {
+                                               uint64_t nstat, uint64_t& st_wedge, uint64_t& st_isz) {
     const uint32_t lane = lane_id();
     uint32_t uj = 0;
     uint64_t tbj = 0, tej = 0;
@@ -419,12 +438,13 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
         uj = V.r_head(k0 + lane);
         V.t_range(uj, tbj, tej);
     }
+    uint64_t pending = __ballot(lane < cnt && tej > tbj);
     {
         const uint64_t cw = uniform64(wave_sum(tej - tbj));
+        const uint32_t nrecv = V.stat_skip_empty() ? (uint32_t)__popcll(pending) : cnt;
         st_wedge += cw;
-        st_isz += cw + (uint64_t)cnt * ns;
+        st_isz += cw + (uint64_t)nrecv * nstat;
     }
-    uint64_t pending = __ballot(lane < cnt && tej > tbj);
     if (pending == 0) return 0;
     const uint32_t* __restrict__ T = V.t_heads();
     const uint32_t l4 = 4 * lane;
@@ -512,7 +532,7 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
 // flushed with one global atomic per neighbour.
 template <int G, bool PLAIN>
 __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uint64_t k0, uint32_t cnt,
-                                                       const ProbeCtx& P, uint64_t ns, uint64_t& st_wedge,
+                                                       const ProbeCtx& P, uint64_t nstat, uint64_t& st_wedge,
                                                        uint64_t& st_isz, PackLds& pk) {
     constexpr uint32_t NQ = KT_WAVE / G;
     const uint32_t lane = lane_id();
@@ -522,14 +542,14 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
         uj = V.r_head(k0 + lane);
         V.t_range(uj, tbj, tej);
     }
-    {
-        const uint64_t cw = uniform64(wave_sum(tej - tbj));
-        st_wedge += cw;
-        st_isz += cw + (uint64_t)cnt * ns;
-    }
     const bool nonempty = lane < cnt && tej > tbj;
     const uint64_t nem = __builtin_amdgcn_ballot_w64(nonempty);
     const uint32_t nne = (uint32_t)__popcll(nem);
+    {
+        const uint64_t cw = uniform64(wave_sum(tej - tbj));
+        st_wedge += cw;
+        st_isz += cw + (uint64_t)(V.stat_skip_empty() ? nne : cnt) * nstat;
+    }
     if (nne == 0) return 0;
     // order[r] = lane of the r-th non-empty list
     uint8_t* order = reinterpret_cast<uint8_t*>(pk.marks);
@@ -614,6 +634,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     rb = uniform64(rb);
     re = uniform64(re);
     const uint64_t ns = se - sb, nr = re - rb;
+    const uint64_t nstat = uniform64(V.stat_len(x, ns));
 
     ProbeCtx P;
     P.keys = keys;
@@ -653,9 +674,9 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     while (k < nr) {
         const uint32_t c = (uint32_t)min<uint64_t>(chunk, nr - k);
         if (GL < KT_WAVE && HASH)
-            tail_hits += scan_chunk_grouped<GL, PLAIN>(V, rb + k, c, P, ns, st_wedge, st_isz, L.pk[wave]);
+            tail_hits += scan_chunk_grouped<GL, PLAIN>(V, rb + k, c, P, nstat, st_wedge, st_isz, L.pk[wave]);
         else
-            tail_hits += scan_chunk<HASH, PLAIN, LIST>(V, rb + k, c, P, ns, st_wedge, st_isz);
+            tail_hits += scan_chunk<HASH, PLAIN, LIST>(V, rb + k, c, P, nstat, st_wedge, st_isz);
         if (NW > 1) {
             uint32_t nk = 0;
             if (lane == 0) nk = atomicAdd(&L.next_u, chunk);

@@ -37,6 +37,8 @@ def check_all_ranks(ctx, g, P, algo, bounds=None, device=False):
         for k in ("wedge_checks", "intersection_size_local", "intersection_size_global", "oriented_edges",
                   "vertices"):
             assert st[k] == hst[k], (k, P, r)
+        for k in ("wedge_checks", "intersection_size_local", "intersection_size_global"):
+            assert st[k] == ost[k], (k, P, r, algo)
         total += loc + t3
         acc += pv
     t, pv_all = ctx.count(g)

@@ -257,6 +257,48 @@ def test_rank_attribution_uneven_ranges():
         assert O.rank_sim(g, P, r, O.DITRIC, bounds=bounds)[:2] == (dl[r], dg[r])
 
 
+def rank_work_counters(g, bounds, r, algorithm):
+    """(wedge_checks, intersection_size_local, intersection_size_global) of
+    rank r as sums over oriented edges (numpy), independent of the oracle's
+    message loops.  Local phase: per tail v and out-neighbour u, |list(v)| +
+    |list(u)| intersected ids and |list(u)| wedge checks.  Message phase: per
+    edge a -> u into the rank from a remote a, |message(a)| + |list(u)| and
+    |list(u)|, where CeTriC's lists and messages are the cut lists (remote
+    out-neighbours) and receivers without one are compacted away."""
+    ooff, oh = O.orient(g)
+    dp = np.diff(ooff).astype(np.int64)
+    tails = np.repeat(np.arange(g.n), dp)
+    owner = np.searchsorted(bounds.astype(np.int64), np.arange(g.n), side="right") - 1
+    mine = owner == r
+    cut = np.bincount(tails, weights=owner[oh] != owner[tails], minlength=g.n).astype(np.int64)
+    into = ~mine[tails] & mine[oh]  # a -> u, a remote, u owned
+    if algorithm == O.DITRIC:
+        loc = mine[tails] & mine[oh]
+        w = dp[oh][loc].sum() + dp[oh][into].sum()
+        return int(w), int((dp[tails] + dp[oh])[loc].sum()), int((dp[tails] + dp[oh])[into].sum())
+    # CeTriC: a remote tail's list holds its owned out-neighbours; an owned
+    # tail sees a ghost head's owned out-neighbours only
+    own_out = np.bincount(tails, weights=mine[oh], minlength=g.n).astype(np.int64)
+    dlist = np.where(mine, dp, own_out)
+    loc = mine[tails] | (into & (own_out[tails] > 0))
+    wl = dlist[oh][loc]
+    isz_l = (dlist[tails][loc] + wl).sum()
+    recv = into & (cut[oh] > 0)
+    return (int(wl.sum() + cut[oh][recv].sum()), int(isz_l),
+            int((cut[tails] + cut[oh])[recv].sum()))
+
+
+@pytest.mark.parametrize("algorithm", [O.CETRIC, O.DITRIC])
+@pytest.mark.parametrize("P", [1, 3, 8])
+def test_rank_work_counters(P, algorithm):
+    g = K.gen_rmat(10, 16, seed=20 + P)
+    bounds = O.even_bounds(g.n, P)
+    for r in range(P):
+        st = O.rank_sim(g, P, r, algorithm, per_vertex=False)[3]
+        got = (st["wedge_checks"], st["intersection_size_local"], st["intersection_size_global"])
+        assert got == rank_work_counters(g, bounds, r, algorithm), (P, r)
+
+
 def test_planted_mistake_is_caught():
     """A plausible attribution mistake (crediting type-3 triangles to the
     owner of the lowest instead of the middle vertex) disagrees with the
