@@ -81,6 +81,8 @@ typedef struct {
 #define KT_STRATEGY_GROUPED16 1u /* groups of 16 lanes walk 4 T lists at a time (short lists) */
 #define KT_STRATEGY_GROUPED8 2u  /* groups of 8 lanes walk 8 T lists at a time (very short lists) */
 #define KT_STRATEGY_GROUPED32 3u /* groups of 32 lanes walk 2 T lists at a time */
+#define KT_STRATEGY_GROUPED4 4u  /* groups of 4 lanes walk 16 T lists at a time */
+#define KT_STRATEGY_GROUPED2 5u  /* groups of 2 lanes walk 32 T lists at a time */
 #define KT_ORDER_DEGREE 0u      /* work queues by degree bucket, largest first */
 #define KT_ORDER_VERTEX 1u      /* work queues by vertex id (locality across a batch's graphs) */
 typedef struct {

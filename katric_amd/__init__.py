@@ -45,6 +45,8 @@ STRATEGY_STREAMED = 0  # KT_STRATEGY_STREAMED
 STRATEGY_GROUPED16 = 1  # KT_STRATEGY_GROUPED16
 STRATEGY_GROUPED8 = 2  # KT_STRATEGY_GROUPED8
 STRATEGY_GROUPED32 = 3  # KT_STRATEGY_GROUPED32
+STRATEGY_GROUPED4 = 4  # KT_STRATEGY_GROUPED4
+STRATEGY_GROUPED2 = 5  # KT_STRATEGY_GROUPED2
 ORDER_DEGREE = 0  # KT_ORDER_DEGREE
 ORDER_VERTEX = 1  # KT_ORDER_VERTEX
 KT_DEVICE_INPUT = 0x1

@@ -238,7 +238,7 @@ extern "C" const char* kt_last_error(const kt_ctx* c) { return c ? c->err : "no 
 extern "C" int kt_set_params(kt_ctx* c, const kt_params* p) {
     if (!c || !p) return KT_EINVAL;
     if (p->wave_max_deg < 2 || p->wave_max_deg > 256 || p->group_max_deg < p->wave_max_deg ||
-        p->group_max_deg > 2048 || p->grid_per_cu < 1 || p->grid_per_cu > 32 || p->strategy > KT_STRATEGY_GROUPED32 || p->order > KT_ORDER_VERTEX)
+        p->group_max_deg > 2048 || p->grid_per_cu < 1 || p->grid_per_cu > 32 || p->strategy > KT_STRATEGY_GROUPED2 || p->order > KT_ORDER_VERTEX)
         return KT_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
     c->prm.wave_max_deg = p->wave_max_deg;

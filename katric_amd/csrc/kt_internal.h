@@ -44,12 +44,20 @@ struct TcParams {
     uint32_t grid_per_cu;     // persistent workgroups per CU
     uint32_t strategy;        // TC_STREAMED or TC_GROUPED* (hash-probe tails)
     uint32_t order;           // TC_BY_DEGREE or TC_BY_VERTEX (work-queue order)
+    uint32_t table_shift = 4; // hash tables hold >= 2^table_shift slots per key where they fit
 };
 
 // How a team walks the T lists of a chunk of neighbours: the whole wave on
 // one list at a time, 256 ids per step (TC_STREAMED, long lists), or groups
-// of 16, 8 or 32 lanes on 4, 8 or 2 lists at a time (TC_GROUPED*, short lists).
-enum : uint32_t { TC_STREAMED = 0, TC_GROUPED16 = 1, TC_GROUPED8 = 2, TC_GROUPED32 = 3 };
+// of G lanes on 64 / G lists at a time (TC_GROUPED<G>, short lists).
+enum : uint32_t {
+    TC_STREAMED = 0,
+    TC_GROUPED16 = 1,
+    TC_GROUPED8 = 2,
+    TC_GROUPED32 = 3,
+    TC_GROUPED4 = 4,
+    TC_GROUPED2 = 5
+};
 // Order of the work queues: tails by degree bucket, largest first, in 13
 // buckets (TC_BY_DEGREE), or by vertex id in two classes -- workgroup tails,
 // then wave tails (TC_BY_VERTEX: tails of one graph of a batch, and their

@@ -254,13 +254,20 @@ struct MsgStep {
 #pragma unroll
         for (unsigned k = 0; k < MSG_E; ++k) c[k] = (p + k < e) ? M.map(x[k]) : 0xffffffffu;
     }
+    // the combined indices stored by a previous pass at the entries' positions
+    __device__ __forceinline__ void load_mapped(const uint32_t* __restrict__ mapped, uint64_t p, uint64_t e) {
+#pragma unroll
+        for (unsigned k = 0; k < MSG_E; ++k) c[k] = (p + k < e) ? mapped[p + k] : 0xffffffffu;
+    }
 };
 
-// Per message: owned / ghost entry counts; sender -> combined index.
+// Per message: owned / ghost entry counts; sender -> combined index.  The
+// combined index of every entry is stored at the entry's position in `mapped`
+// (one streaming write instead of a second round of rank-select gathers).
 __global__ void k_msg_count(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint32_t* __restrict__ src,
                             const uint64_t* __restrict__ moff, const uint32_t* __restrict__ mheads,
                             uint64_t* __restrict__ kept, uint32_t* __restrict__ nown, uint32_t* __restrict__ tail_id,
-                            uint32_t* __restrict__ err) {
+                            uint32_t* __restrict__ mapped, uint32_t* __restrict__ err) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / MSG_G;
     const unsigned gl = lane_id() & (MSG_G - 1);
@@ -278,6 +285,7 @@ __global__ void k_msg_count(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint
 #pragma unroll
                 for (unsigned k = 0; k < MSG_E; ++k) {
                     if (p + k >= e) continue;
+                    mapped[p + k] = s.c[k];
                     const uint32_t nx = (k + 1 < MSG_E) ? s.x[k + 1] : xn;
                     if (p + k + 1 < e && nx <= s.x[k]) flags |= KT_ERR_MESSAGE;
                     own += s.c[k] < M.nl ? 1u : 0u;
@@ -299,8 +307,8 @@ __global__ void k_msg_count(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint
 }
 
 // Per message: owned entries then ghost entries, each in message order.
-__global__ void k_msg_write(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint64_t* __restrict__ moff,
-                            const uint32_t* __restrict__ mheads, const uint64_t* __restrict__ coff_msg,
+__global__ void k_msg_write(uint32_t nl, uint32_t nmsg, uint64_t mtot, const uint64_t* __restrict__ moff,
+                            const uint32_t* __restrict__ mapped, const uint64_t* __restrict__ coff_msg,
                             const uint32_t* __restrict__ nown, uint32_t* __restrict__ out,
                             uint64_t* __restrict__ split) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -314,19 +322,19 @@ __global__ void k_msg_write(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint
         for (uint64_t p0 = b; p0 < e; p0 += MSG_G * MSG_E) {
             const uint64_t p = p0 + MSG_E * gl;
             MsgStep s;
-            s.load(M, mheads, p, e);
+            s.load_mapped(mapped, p, e);
             uint32_t o = 0, h = 0;
 #pragma unroll
             for (unsigned k = 0; k < MSG_E; ++k) {
-                o += s.c[k] < M.nl ? 1u : 0u;
-                h += (s.c[k] != 0xffffffffu && s.c[k] >= M.nl) ? 1u : 0u;
+                o += s.c[k] < nl ? 1u : 0u;
+                h += (s.c[k] != 0xffffffffu && s.c[k] >= nl) ? 1u : 0u;
             }
             const uint32_t oi = group_incl_sum(o), hi = group_incl_sum(h);
             const uint32_t ot = __shfl(oi, MSG_G - 1, MSG_G), ht = __shfl(hi, MSG_G - 1, MSG_G);
             uint64_t qo = po + (oi - o), qg = pg + (hi - h);
 #pragma unroll
             for (unsigned k = 0; k < MSG_E; ++k) {
-                if (s.c[k] < M.nl)
+                if (s.c[k] < nl)
                     out[qo++] = s.c[k];
                 else if (s.c[k] != 0xffffffffu)
                     out[qg++] = s.c[k];
@@ -371,6 +379,7 @@ struct ShardBufs {
     uint32_t* mtail;
     uint64_t* msplit;
     uint32_t* mheads2;
+    uint32_t* mmapped;
     uint64_t* scan_tmp;
     uint64_t* tc_ws;
     uint64_t* pv;
@@ -408,6 +417,7 @@ uint64_t shard_layout(void* base, const kt_shard_in* in, uint64_t nnz, uint64_t 
     B.mtail = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 4));
     B.msplit = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(in->n_msg, 1) * 8));
     B.mheads2 = static_cast<uint32_t*>(cv.take((mtot + 4) * 4));
+    B.mmapped = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(mtot, 1) * 4));
     const uint64_t n_all = in->range_bounds[in->num_ranks];
     const uint64_t rs_words = (n_all + 31) / 32;
     B.scan_tmp = static_cast<uint64_t*>(
@@ -541,10 +551,10 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
     // messages
     if (in->n_msg) {
         k_msg_count<<<(unsigned)std::min<uint64_t>(((uint64_t)in->n_msg * MSG_G + 255) / 256, 8192), 256, 0, st>>>(
-            M, in->n_msg, mtot, B.msg_src, B.msg_off, B.msg_heads, B.mkept, B.mnown, B.mtail, err);
+            M, in->n_msg, mtot, B.msg_src, B.msg_off, B.msg_heads, B.mkept, B.mnown, B.mtail, B.mmapped, err);
         scan_u64(B.mkept, B.moff, in->n_msg, B.scan_tmp, st);
         k_msg_write<<<(unsigned)std::min<uint64_t>(((uint64_t)in->n_msg * MSG_G + 255) / 256, 8192), 256, 0, st>>>(
-            M, in->n_msg, mtot, B.msg_off, B.msg_heads, B.moff, B.mnown, B.mheads2, B.msplit);
+            M.nl, in->n_msg, mtot, B.msg_off, B.mmapped, B.moff, B.mnown, B.mheads2, B.msplit);
     }
     rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st), "D2H flags");
     if (rc) return rc;

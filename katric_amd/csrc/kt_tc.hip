@@ -98,6 +98,11 @@ struct View {
         }
     }
     __device__ __forceinline__ const uint32_t* t_heads() const { return PLAIN ? v.sheads : v.theads; }
+    // entry i of the walk: the neighbour u and the bounds of T(u)
+    __device__ __forceinline__ void walk_entry(uint64_t i, uint32_t& u, uint64_t& b, uint64_t& e) const {
+        u = r_head(i);
+        t_range(u, b, e);
+    }
     __device__ __forceinline__ uint32_t tail_pv(uint32_t x) const {
         return (PLAIN || v.tail_id == nullptr) ? x : v.tail_id[x];
     }
@@ -159,8 +164,9 @@ __global__ void __launch_bounds__(256) k_tc_bin_count(View<PLAIN> V, TcParams pr
         // intersection in the reference's counters (cetric_edge_iterator.h:279-281)
         if (PLAIN && s == 1) {
             uint64_t sb, se, tb, te;
+            uint32_t u;
             V.s_range(x, sb, se);
-            V.t_range(V.r_head(sb), tb, te);
+            V.walk_entry(sb, u, tb, te);
             wedge += te - tb;
             isz += 1 + (te - tb);
         }
@@ -172,7 +178,8 @@ __global__ void __launch_bounds__(256) k_tc_bin_count(View<PLAIN> V, TcParams pr
             const uint64_t len = V.stat_len(x, 0);
             for (uint64_t i = rb; i < re; ++i) {
                 uint64_t tb, te;
-                V.t_range(V.r_head(i), tb, te);
+                uint32_t u;
+                V.walk_entry(i, u, tb, te);
                 if (te == tb && V.stat_skip_empty()) continue;
                 wedge += te - tb;
                 isz += len + (te - tb);
@@ -435,8 +442,7 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
     uint32_t uj = 0;
     uint64_t tbj = 0, tej = 0;
     if (lane < cnt) {
-        uj = V.r_head(k0 + lane);
-        V.t_range(uj, tbj, tej);
+        V.walk_entry(k0 + lane, uj, tbj, tej);
     }
     uint64_t pending = __ballot(lane < cnt && tej > tbj);
     {
@@ -539,8 +545,7 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
     uint32_t uj = 0;
     uint64_t tbj = 0, tej = 0;
     if (lane < cnt) {
-        uj = V.r_head(k0 + lane);
-        V.t_range(uj, tbj, tej);
+        V.walk_entry(k0 + lane, uj, tbj, tej);
     }
     const bool nonempty = lane < cnt && tej > tbj;
     const uint64_t nem = __builtin_amdgcn_ballot_w64(nonempty);
@@ -619,7 +624,8 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
 // Process tail x with a team of NW waves (NW == 1: this wave alone).
 template <int NW, bool HASH, bool PLAIN, int GL, bool LIST, class LdsT>
 __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, LdsT& L, uint32_t* keys,
-                                             uint32_t* cnt, uint32_t cap_slots, uint32_t chunk, const TcOut& out,
+                                             uint32_t* cnt, uint32_t cap_slots, uint32_t shift, uint32_t chunk,
+                                             const TcOut& out,
                                              uint64_t& my_total, uint64_t& st_wedge, uint64_t& st_isz) {
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / KT_WAVE;
@@ -646,7 +652,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, L
     P.pv = out.per_vertex;
     uint32_t slots = 0;
     if (HASH) {
-        uint32_t logS = ceil_log2(16 * ns);  // load <= 1/16 unless capped
+        uint32_t logS = ceil_log2(ns << shift);  // load <= 2^-shift unless capped
         if (logS < MIN_LOGS) logS = MIN_LOGS;
         slots = 1u << logS;
         if (slots > cap_slots) {  // bins guarantee cap_slots >= 2 * ns
@@ -750,7 +756,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             const uint32_t i = L.bcast;
             __syncthreads();
             if (i >= cnt) break;
-            team_process<TC_WAVES, true, PLAIN, GL, LIST>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.team_chunk,
+            team_process<TC_WAVES, true, PLAIN, GL, LIST>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.table_shift, prm.team_chunk,
                                                 out, my_total, st_wedge, st_isz);
         }
     }
@@ -778,7 +784,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
                 if (i >= cnt) break;
                 const uint32_t e = min(cnt, i + grab);
                 for (; i < e; ++i)
-                    team_process<1, true, PLAIN, GL, LIST>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, KT_WAVE, out,
+                    team_process<1, true, PLAIN, GL, LIST>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, prm.table_shift, KT_WAVE, out,
                                                  my_total, st_wedge, st_isz);
             }
         }
@@ -827,7 +833,7 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
         const uint32_t i = L.bcast;
         __syncthreads();
         if (i >= cnt) break;
-        team_process<TC_WAVES, false, PLAIN, KT_WAVE, LIST>(V, list[base + i], L, nullptr, nullptr, 0, prm.team_chunk, out,
+        team_process<TC_WAVES, false, PLAIN, KT_WAVE, LIST>(V, list[base + i], L, nullptr, nullptr, 0, 0, prm.team_chunk, out,
                                              my_total, st_wedge, st_isz);
     }
     if (lane_id() == 0) {
@@ -860,7 +866,8 @@ __global__ void __launch_bounds__(BIG_THREADS) k_tc_big(View<PLAIN> V, TcOut out
         __syncthreads();
         if (i >= cnt) break;
         team_process<BIG_WAVES, true, PLAIN, KT_WAVE, LIST>(V, list[base + i], L, big_table, big_table + BIG_SLOTS,
-                                                      BIG_SLOTS, prm.team_chunk, out, my_total, st_wedge, st_isz);
+                                                      BIG_SLOTS, prm.table_shift, prm.team_chunk, out, my_total,
+                                                      st_wedge, st_isz);
     }
     if (lane_id() == 0) {
         if (my_total) atomic_add_u64(&out.totals[0], my_total);
@@ -931,6 +938,8 @@ static hipError_t launch_impl(const TcView& view, const TcOut& out, const TcPara
         case TC_GROUPED32: k_tc_main<PLAIN, 32, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
         case TC_GROUPED16: k_tc_main<PLAIN, 16, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
         case TC_GROUPED8: k_tc_main<PLAIN, 8, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+        case TC_GROUPED4: k_tc_main<PLAIN, 4, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
+        case TC_GROUPED2: k_tc_main<PLAIN, 2, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
         default: k_tc_main<PLAIN, KT_WAVE, false><<<grid, TC_THREADS, 0, st>>>(V, out, p, q, list, stats); break;
     }
     if (ev_end) (void)hipEventRecord(ev_end, st);

@@ -60,7 +60,7 @@ def batch_key(num_graphs: int, offsets: np.ndarray) -> str:
 
 STAGES = [
     # stage 1: how lists are walked and in which order tails are taken
-    {"strategy": [0, 1, 2, 3], "order": [0, 1]},
+    {"strategy": [0, 1, 2, 3, 4, 5], "order": [0, 1]},
     # stage 2: the bin thresholds and the grid, under the best stage-1 choice
     {"wave_max_deg": [32, 64, 128, 256], "group_max_deg": [1024, 2048], "grid_per_cu": [2, 3, 4]},
 ]

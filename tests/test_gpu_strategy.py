@@ -16,7 +16,8 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-STRATEGIES = [K.STRATEGY_STREAMED, K.STRATEGY_GROUPED16, K.STRATEGY_GROUPED8, K.STRATEGY_GROUPED32]
+STRATEGIES = [K.STRATEGY_STREAMED, K.STRATEGY_GROUPED16, K.STRATEGY_GROUPED8, K.STRATEGY_GROUPED32,
+              K.STRATEGY_GROUPED4, K.STRATEGY_GROUPED2]
 # (wave_max_deg, group_max_deg, grid_per_cu)
 BINS = [(128, 2048, 4), (8, 256, 1), (256, 512, 2), (2, 2048, 3)]
 
