@@ -191,7 +191,7 @@ def run_shard(args):
                         f"rank {args.rank}: local CSR + ghost degrees + N+cut messages resident in HBM -> "
                         f"local / type-3 totals + per-vertex uint64 in HBM",
             "n": n, "m": m, "owned": nl, "ghosts": ng, "messages": nmsg, "message_ids": mtot,
-            "oriented_shard_edges": Mk, "W": W, "edges_counted": units,
+            "oriented_shard_edges": Mk, "W": W, "wedge_checks": st["wedge_checks"], "edges_counted": units,
             "local_triangles": loc, "type3_triangles": t3,
             "device_ms": tm["total_ms"], "prep_ms": tm["prep_ms"], "count_ms": tm["count_ms"],
             "pipeline_algorithmic_GBs": (b_pre + b_count) / (ms_per_step * 1e-3) / 1e9,
