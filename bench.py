@@ -43,7 +43,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # per dispatch).  A plain 1 GiB device copy reads FETCH_SIZE = 524,308 KiB and
 # WRITE_SIZE = 1,048,576 KiB on MI355X, so FETCH_SIZE is scaled by 2.000 and
 # WRITE_SIZE by 1.000: 57,571,680 KiB x 2.000 + 2,829,838 KiB = 120.80 GB.
-MEASURED_TRAFFIC = {("plain", 22, 16, 1): 120.80e9}
+# Shard (RMAT-26 ef16 seed 1, 16 ranks, rank 0, CeTriC), the count phase's
+# intersection launches (k_tc_main + k_tc_big, local and contracted; FETCH_SIZE
+# also over binning and k_tc_huge): FETCH_SIZE 264,416,009 KiB x 2.000 + WRITE_SIZE 11,842,678 KiB = 553.65 GB.
+MEASURED_TRAFFIC = {("plain", 22, 16, 1): 120.80e9, ("shard-cetric", 26, 16, 1, 16, 0): 553.65e9}
 
 
 
@@ -198,7 +201,9 @@ def run_shard(args):
         },
         "roofline": {
             "kernel": "k_tc_main (local + contracted phase)", "bound": "hbm", "achieved": achieved,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None if (args.params or args.ditric) else MEASURED_TRAFFIC.get(
+                ("shard-cetric", args.scale, args.edge_factor, args.seed, args.ranks, args.rank)),
             "kernel_ms": c_ms,
         },
         "cpu_baseline": None,
