@@ -621,26 +621,36 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
     return wave_total_dpp(nh);
 }
 
-// Process tail x with a team of NW waves (NW == 1: this wave alone).
+// Bounds of a tail's S and R ranges and its counter length.
+struct TailBounds {
+    uint64_t sb, se, rb, re, nstat;
+};
+
+template <bool PLAIN>
+__device__ __forceinline__ TailBounds tail_bounds(const View<PLAIN>& V, uint32_t x) {
+    TailBounds t;
+    V.s_range(x, t.sb, t.se);
+    V.r_range(x, t.rb, t.re);
+    t.nstat = V.stat_len(x, t.se - t.sb);
+    return t;
+}
+
+// Process tail x, whose bounds are tb, with a team of NW waves (NW == 1:
+// this wave alone).
 template <int NW, bool HASH, bool PLAIN, int GL, bool LIST, class LdsT>
-__device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, LdsT& L, uint32_t* keys,
-                                             uint32_t* cnt, uint32_t cap_slots, uint32_t shift, uint32_t chunk,
-                                             const TcOut& out,
+__device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, const TailBounds& tb, LdsT& L,
+                                             uint32_t* keys, uint32_t* cnt, uint32_t cap_slots, uint32_t shift,
+                                             uint32_t chunk, const TcOut& out,
                                              uint64_t& my_total, uint64_t& st_wedge, uint64_t& st_isz) {
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / KT_WAVE;
     const unsigned twave = NW > 1 ? wave : 0;
     const unsigned tid = twave * KT_WAVE + lane;
 
-    uint64_t sb, se, rb, re;
-    V.s_range(x, sb, se);
-    V.r_range(x, rb, re);
-    sb = uniform64(sb);
-    se = uniform64(se);
-    rb = uniform64(rb);
-    re = uniform64(re);
+    const uint64_t sb = uniform64(tb.sb), se = uniform64(tb.se);
+    const uint64_t rb = uniform64(tb.rb), re = uniform64(tb.re);
     const uint64_t ns = se - sb, nr = re - rb;
-    const uint64_t nstat = uniform64(V.stat_len(x, ns));
+    const uint64_t nstat = uniform64(tb.nstat);
 
     ProbeCtx P;
     P.keys = keys;
@@ -756,8 +766,10 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             const uint32_t i = L.bcast;
             __syncthreads();
             if (i >= cnt) break;
-            team_process<TC_WAVES, true, PLAIN, GL, LIST>(V, list[base + i], L, L.keys, L.cnt, GROUP_SLOTS, prm.table_shift, prm.team_chunk,
-                                                out, my_total, st_wedge, st_isz);
+            const uint32_t x = list[base + i];
+            team_process<TC_WAVES, true, PLAIN, GL, LIST>(V, x, tail_bounds(V, x), L, L.keys, L.cnt, GROUP_SLOTS,
+                                                          prm.table_shift, prm.team_chunk, out, my_total, st_wedge,
+                                                          st_isz);
         }
     }
     // wave phase: independent waves with private tables; by degree, largest
@@ -782,10 +794,22 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
                             : atomicAdd(pull, grab);
                 i = uniform(i);
                 if (i >= cnt) break;
-                const uint32_t e = min(cnt, i + grab);
-                for (; i < e; ++i)
-                    team_process<1, true, PLAIN, GL, LIST>(V, list[base + i], L, keys, tcnt, WAVE_SLOTS, prm.table_shift, KT_WAVE, out,
-                                                 my_total, st_wedge, st_isz);
+                // the grabbed tails (grab <= 64) and their bounds, loaded by
+                // one lane each before the first is processed: one round
+                // trip for the batch instead of two per tail
+                const uint32_t ng = min(cnt, i + grab) - i;
+                uint32_t xl = 0;
+                TailBounds bl{0, 0, 0, 0, 0};
+                if (lane < ng) {
+                    xl = list[base + i + lane];
+                    bl = tail_bounds(V, xl);
+                }
+                for (uint32_t j = 0; j < ng; ++j) {
+                    const TailBounds tb{readlane64(bl.sb, j), readlane64(bl.se, j), readlane64(bl.rb, j),
+                                        readlane64(bl.re, j), readlane64(bl.nstat, j)};
+                    team_process<1, true, PLAIN, GL, LIST>(V, readlane32(xl, j), tb, L, keys, tcnt, WAVE_SLOTS,
+                                                           prm.table_shift, KT_WAVE, out, my_total, st_wedge, st_isz);
+                }
             }
         }
     }
@@ -833,8 +857,9 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
         const uint32_t i = L.bcast;
         __syncthreads();
         if (i >= cnt) break;
-        team_process<TC_WAVES, false, PLAIN, KT_WAVE, LIST>(V, list[base + i], L, nullptr, nullptr, 0, 0, prm.team_chunk, out,
-                                             my_total, st_wedge, st_isz);
+        const uint32_t x = list[base + i];
+        team_process<TC_WAVES, false, PLAIN, KT_WAVE, LIST>(V, x, tail_bounds(V, x), L, nullptr, nullptr, 0, 0,
+                                                            prm.team_chunk, out, my_total, st_wedge, st_isz);
     }
     if (lane_id() == 0) {
         if (my_total) atomic_add_u64(&out.totals[0], my_total);
@@ -865,9 +890,10 @@ __global__ void __launch_bounds__(BIG_THREADS) k_tc_big(View<PLAIN> V, TcOut out
         const uint32_t i = L.bcast;
         __syncthreads();
         if (i >= cnt) break;
-        team_process<BIG_WAVES, true, PLAIN, KT_WAVE, LIST>(V, list[base + i], L, big_table, big_table + BIG_SLOTS,
-                                                      BIG_SLOTS, prm.table_shift, prm.team_chunk, out, my_total,
-                                                      st_wedge, st_isz);
+        const uint32_t x = list[base + i];
+        team_process<BIG_WAVES, true, PLAIN, KT_WAVE, LIST>(V, x, tail_bounds(V, x), L, big_table,
+                                                            big_table + BIG_SLOTS, BIG_SLOTS, prm.table_shift,
+                                                            prm.team_chunk, out, my_total, st_wedge, st_isz);
     }
     if (lane_id() == 0) {
         if (my_total) atomic_add_u64(&out.totals[0], my_total);
