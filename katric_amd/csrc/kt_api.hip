@@ -194,7 +194,7 @@ extern "C" int kt_create(kt_ctx** out, int device, uint64_t hbm_budget_bytes) {
     }
     c->prm.wave_max_deg = 128;
     c->prm.group_max_deg = 2048;
-    c->prm.team_chunk = 16;
+    c->prm.team_chunk = 8;
     c->prm.grid_per_cu = 4;
     c->prm.strategy = TC_STREAMED;
     c->prm.order = TC_BY_DEGREE;
