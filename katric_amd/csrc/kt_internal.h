@@ -45,6 +45,7 @@ struct TcParams {
     uint32_t strategy;        // TC_STREAMED or TC_GROUPED* (hash-probe tails)
     uint32_t order;           // TC_BY_DEGREE or TC_BY_VERTEX (work-queue order)
     uint32_t table_shift = 4; // hash tables hold >= 2^table_shift slots per key where they fit
+    uint32_t wave_grab = 32;  // wave-bin tails a wave takes per queue access (1..64), vertex order
 };
 
 // How a team walks the T lists of a chunk of neighbours: the whole wave on

@@ -784,8 +784,10 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             if (cnt == 0) continue;
             const uint32_t base = bucket_base(q, b);
             uint32_t* pull = &q[Q_PULL + Q_STRIDE * b];
-            // more tails per grab for the small buckets (less queue traffic)
-            const uint32_t grab = (b <= 3 || by_vertex) ? 4 * WAVE_GRAB : (b <= 5 ? 2 * WAVE_GRAB : WAVE_GRAB);
+            // more tails per grab for the small buckets (less queue traffic);
+            // in vertex order, the grab set per launch
+            const uint32_t grab = by_vertex ? min(max(prm.wave_grab, 1u), (uint32_t)KT_WAVE)
+                                            : (b <= 3 ? 4 * WAVE_GRAB : (b <= 5 ? 2 * WAVE_GRAB : WAVE_GRAB));
             while (true) {
                 uint32_t i = 0;
                 if (lane == 0)
