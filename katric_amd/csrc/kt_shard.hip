@@ -621,12 +621,13 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
         // them idles, so tails up to the largest wave table stay with one
         // wave (measured on RMAT-22 and RMAT-26 / 16 ranks); the workgroup
         // teams on the remaining (large) message tails share their receivers
-        // 4 at a time, since a few long cut lists per message make coarser
-        // grabs leave waves idle at the end (RMAT-26 / 16: contracted phase
-        // 64.6 ms with 8 per grab, 58.4 ms with 4)
+        // 4 (CeTriC) or 2 (DiTriC) at a time, since a few long lists per
+        // message make coarser grabs leave waves idle at the end (RMAT-26 /
+        // 16: CeTriC contracted phase 64.6 ms with 8 per grab, 58.4 ms with 4;
+        // DiTriC count 76.7 / 70.3 / 68.9 ms with 8 / 4 / 2)
         TcParams pm = c->prm;
         pm.wave_max_deg = std::max<uint32_t>(pm.wave_max_deg, 256u);
-        pm.team_chunk = std::min<uint32_t>(pm.team_chunk, 4u);
+        pm.team_chunk = std::min<uint32_t>(pm.team_chunk, cetric ? 4u : 2u);
         rc = c->hip_check(tc_launch(v, o, pm, B.tc_ws, c->num_cus, st, nullptr, nullptr), "intersection launch");
         if (rc) return rc;
         rc = c->hip_check(hipMemcpyAsync(c->host_small + 12, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
