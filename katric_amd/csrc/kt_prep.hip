@@ -340,46 +340,67 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint64_t* __res
     hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Stable scatter of one 4096-key tile by the digit at `shift`: the tile is
+// first ranked into digit order in LDS (rounds of 256 keys in index order;
+// within a round, waves in order and lanes in order, so equal digits keep
+// their input order), then written out from LDS in that order, so that
+// consecutive threads store consecutive positions of one digit's run.
 __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const uint64_t* __restrict__ in, uint64_t len,
-                                                               int shift, const uint64_t* __restrict__ hist_off,
+                                                               int shift, const uint64_t* __restrict__ hist,
+                                                               const uint64_t* __restrict__ hist_off,
                                                                uint32_t ntiles, uint64_t* __restrict__ out) {
-    __shared__ uint64_t run[256];
+    __shared__ uint64_t sk[RS_TILE];
+    __shared__ uint64_t gbase[256];                   // global start of this tile's run of digit d
+    __shared__ uint32_t dstart[256];                  // local start of digit d in the ranked tile
+    __shared__ uint32_t run[256];                     // keys of digit d placed so far
     __shared__ uint32_t wcnt[RS_THREADS / KT_WAVE][256];
-    const unsigned w = threadIdx.x / KT_WAVE;
-    run[threadIdx.x] = hist_off[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
-    uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    __shared__ uint64_t lw[RS_THREADS / KT_WAVE];
+    const unsigned t = threadIdx.x, w = t / KT_WAVE;
+    const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+    const uint32_t nt = (uint32_t)min<uint64_t>(RS_TILE, len - base);
+    {
+        const uint64_t c = hist[(uint64_t)t * ntiles + blockIdx.x];
+        uint64_t tot;
+        dstart[t] = (uint32_t)block_excl_sum(c, lw, &tot);
+        gbase[t] = hist_off[(uint64_t)t * ntiles + blockIdx.x];
+        run[t] = 0;
+    }
     for (int i = 0; i < RS_ROUNDS; ++i) {
-        for (int q = 0; q < RS_THREADS / KT_WAVE; ++q) wcnt[q][threadIdx.x] = 0;
+        for (int q = 0; q < RS_THREADS / KT_WAVE; ++q) wcnt[q][t] = 0;
         __syncthreads();
-        uint64_t idx = base + (uint64_t)i * RS_THREADS + threadIdx.x;
-        bool valid = idx < len;
-        uint64_t k = valid ? in[idx] : 0;
-        uint32_t d = (uint32_t)((k >> shift) & 255);
+        const uint32_t li = (uint32_t)i * RS_THREADS + t;
+        const bool valid = li < nt;
+        const uint64_t k = valid ? in[base + li] : 0;
+        const uint32_t d = (uint32_t)((k >> shift) & 255);
         // lanes of this wave holding the same digit
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            uint64_t bb = __ballot((d >> b) & 1);
+            const uint64_t bb = __ballot((d >> b) & 1);
             peers &= ((d >> b) & 1) ? bb : ~bb;
         }
-        uint32_t wrank = __popcll(peers & lanemask_lt());
-        bool leader = valid && wrank == 0;
-        if (leader) wcnt[w][d] = __popcll(peers);
+        const uint32_t wrank = __popcll(peers & lanemask_lt());
+        if (valid && wrank == 0) wcnt[w][d] = __popcll(peers);
         __syncthreads();
+        // per digit: waves' offsets within this round, and the digit's new total
         {
-            uint64_t r = run[threadIdx.x];
+            uint32_t r = run[t];
 #pragma unroll
             for (int q = 0; q < RS_THREADS / KT_WAVE; ++q) {
-                uint32_t c = wcnt[q][threadIdx.x];
-                wcnt[q][threadIdx.x] = (uint32_t)(r - run[threadIdx.x]);
+                const uint32_t c = wcnt[q][t];
+                wcnt[q][t] = r;
                 r += c;
             }
-            __syncthreads();
-            if (valid) out[run[d] + wcnt[w][d] + wrank] = k;
-            __syncthreads();
-            run[threadIdx.x] = r;
+            run[t] = r;
         }
         __syncthreads();
+        if (valid) sk[dstart[d] + wcnt[w][d] + wrank] = k;
+        __syncthreads();  // wcnt is cleared for the next round
+    }
+    for (uint32_t p = t; p < nt; p += RS_THREADS) {
+        const uint64_t k = sk[p];
+        const uint32_t d = (uint32_t)((k >> shift) & 255);
+        out[gbase[d] + (p - dstart[d])] = k;
     }
 }
 
@@ -405,7 +426,7 @@ void radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t len, int bits, uint6
         int shift = 8 * p;
         k_radix_hist<<<ntiles, RS_THREADS, 0, st>>>(src, len, shift, hist, ntiles);
         scan_u64(hist, hoff, h, stmp, st);
-        k_radix_scatter<<<ntiles, RS_THREADS, 0, st>>>(src, len, shift, hoff, ntiles, dst);
+        k_radix_scatter<<<ntiles, RS_THREADS, 0, st>>>(src, len, shift, hist, hoff, ntiles, dst);
         uint64_t* t = src;
         src = dst;
         dst = t;
