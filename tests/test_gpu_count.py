@@ -54,6 +54,22 @@ def test_orient_matches_oracle(ctx):
     np.testing.assert_array_equal(og.heads, oheads)
 
 
+def test_orient_unsorted_many_tiles(ctx):
+    # shuffled rows of a graph with ~2 x 10^5 oriented edges: the radix sort
+    # runs over dozens of 4096-key tiles (the last one partial) and must give
+    # exactly the oracle's ascending oriented lists
+    g = K.gen_rmat(13, 16, seed=7)
+    rng = np.random.default_rng(8)
+    heads = g.heads.copy()
+    for v in range(g.n):
+        a, b = int(g.offsets[v]), int(g.offsets[v + 1])
+        heads[a:b] = rng.permutation(heads[a:b])
+    og = ctx.orient(K.Graph(g.n, g.offsets, heads))
+    ooff, oheads = O.orient(g)
+    np.testing.assert_array_equal(og.offsets, ooff)
+    np.testing.assert_array_equal(og.heads, oheads)
+
+
 def test_unsorted_input(ctx):
     # metis-sample's lists are not ascending in the file
     g = K.read_metis(golden("metis-sample.metis"))
