@@ -185,16 +185,18 @@ __global__ void k_pd_sinfo(uint32_t nmsg, const uint32_t* __restrict__ src, cons
     }
 }
 
-// Tiles of the virtual slot space [0, V).  Counting pass (!EMIT): per-tile
-// kept counts, validation and order flags.  EMIT: kept ids to mheads at
-// tile_off[tile] + rank in the tile, and moff[i] = the output position of
-// sender i's first virtual slot.
+// Tiles of the virtual slot space [0, V).  Counting pass (!EMIT): keep bits
+// (one word per 64 slots, to vkeep), per-tile kept counts, validation and
+// order flags.  EMIT: kept ids, by the counting pass's bits (no degree
+// gathers), to mheads at tile_off[tile] + rank in the tile, and moff[i] =
+// the output position of sender i's first virtual slot.
 template <bool EMIT>
 __global__ void __launch_bounds__(PD_THREADS)
     k_pd_msgs(uint64_t V, uint32_t nmsg, const uint64_t* __restrict__ voff, const uint32_t* __restrict__ tbnd,
               const SenderInfo* __restrict__ si, const uint32_t* __restrict__ heads, const uint32_t* __restrict__ deg,
-              uint32_t n, uint64_t* __restrict__ tile_cnt, const uint64_t* __restrict__ tile_off,
-              uint32_t* __restrict__ mheads, uint64_t* __restrict__ moff, uint32_t* __restrict__ err) {
+              uint32_t n, uint64_t* __restrict__ vkeep, uint64_t* __restrict__ tile_cnt,
+              const uint64_t* __restrict__ tile_off, uint32_t* __restrict__ mheads, uint64_t* __restrict__ moff,
+              uint32_t* __restrict__ err) {
     constexpr int NW = PD_THREADS / KT_WAVE;
     __shared__ uint32_t marks[PD_TILE];
     __shared__ uint32_t wmax[NW];
@@ -216,26 +218,35 @@ __global__ void __launch_bounds__(PD_THREADS)
         ps[k] = s->offa + (j - s->vbeg);
         xs[k] = j < tend ? heads[ps[k]] : 0u;
     }
-#pragma unroll
-    for (int k = 0; k < PD_ITEMS; ++k) dx[k] = (tile0 + k * PD_THREADS + tid < tend && xs[k] < n) ? deg[xs[k]] : 0u;
     uint32_t flags = 0;
     uint64_t bits[PD_ITEMS];
+    if constexpr (!EMIT) {
+#pragma unroll
+        for (int k = 0; k < PD_ITEMS; ++k)
+            dx[k] = (tile0 + k * PD_THREADS + tid < tend && xs[k] < n) ? deg[xs[k]] : 0u;
+    }
 #pragma unroll
     for (int k = 0; k < PD_ITEMS; ++k) {
         const unsigned idx = k * PD_THREADS + tid;
         const uint64_t j = tile0 + idx;
-        bool keep = false;
-        if (j < tend) {
-            const SenderInfo* s = si + (lo + marks[idx]);
-            const uint32_t a = s->a, x = xs[k];
-            if (x >= n) {
-                flags |= KT_ERR_HEAD_RANGE;
-            } else {
-                keep = before(s->da, a, dx[k], x) && !(x >= s->lo && x < s->hi);
-                if (!EMIT && ps[k] + 1 < s->offa + s->da && heads[ps[k] + 1] <= x) flags |= KT_FLAG_UNSORTED;
+        const uint64_t w0 = tile0 + k * PD_THREADS + (tid & ~63u);  // first slot of this wave's word
+        if constexpr (EMIT) {
+            bits[k] = w0 < tend ? vkeep[w0 / 64] : 0;
+        } else {
+            bool keep = false;
+            if (j < tend) {
+                const SenderInfo* s = si + (lo + marks[idx]);
+                const uint32_t a = s->a, x = xs[k];
+                if (x >= n) {
+                    flags |= KT_ERR_HEAD_RANGE;
+                } else {
+                    keep = before(s->da, a, dx[k], x) && !(x >= s->lo && x < s->hi);
+                    if (ps[k] + 1 < s->offa + s->da && heads[ps[k] + 1] <= x) flags |= KT_FLAG_UNSORTED;
+                }
             }
+            bits[k] = __ballot(keep);
+            if (lane_id() == 0 && w0 < tend) vkeep[w0 / 64] = bits[k];
         }
-        bits[k] = __ballot(keep);
         if (lane_id() == 0) wcnt[k * NW + wave] = (uint32_t)__popcll(bits[k]);
     }
     if (!EMIT && flags) atomicOr(err, flags);
@@ -323,6 +334,7 @@ struct ProdBufs {
     uint64_t* stmp;
     uint32_t* tbnd;
     uint64_t* tcnt;  // per-tile kept counts of the virtual slot space, then their scan
+    uint64_t* vkeep; // keep bits of the virtual slot space, one word per 64 slots
     uint64_t* loff;
     uint32_t* ghost_ids;
     uint64_t* ghost_deg;
@@ -350,6 +362,7 @@ uint64_t prod_layout(void* base, uint32_t n, uint64_t nnz, uint32_t P, uint32_t 
     B.stmp = static_cast<uint64_t*>(cv.take(scan_tmp_words(std::max(std::max<uint64_t>(W, ng_max), tiles) + 1) * 8));
     B.tbnd = static_cast<uint32_t*>(cv.take(2 * tiles * 4));
     B.tcnt = static_cast<uint64_t*>(cv.take((tiles + 1) * 8));
+    B.vkeep = static_cast<uint64_t*>(cv.take(tiles * (PD_TILE / 64) * 8));
     B.loff = static_cast<uint64_t*>(cv.take(((uint64_t)nl + 1) * 8));
     B.ghost_ids = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(ng_max, 1) * 4));
     B.ghost_deg = static_cast<uint64_t*>(cv.take(std::max<uint64_t>(ng_max, 1) * 8));
@@ -470,7 +483,8 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
                                                    cetric ? 1 : 0, B.sinfo);
         tile_bounds_launch((uint32_t)nmsg, V, PD_TILE, B.voff, B.tbnd, st);
         k_pd_msgs<false><<<(unsigned)tiles_v, PD_THREADS, 0, st>>>(V, (uint32_t)nmsg, B.voff, B.tbnd, B.sinfo, hd,
-                                                                   B.deg, n, B.tcnt, nullptr, nullptr, nullptr, err);
+                                                                   B.deg, n, B.vkeep, B.tcnt, nullptr, nullptr,
+                                                                   nullptr, err);
         scan_u64(B.tcnt, B.tcnt, tiles_v, B.stmp, st);
         rc = c->hip_check(hipMemcpyAsync(B.small + 1, B.tcnt + tiles_v, 8, hipMemcpyDeviceToDevice, st), "D2D total");
         if (!rc) rc = read_small(c, B.small, 2, "message lengths");
@@ -492,7 +506,8 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
         cv = Carver{c->aux[1], 0};
         mheads = static_cast<uint32_t*>(cv.take(std::max<uint64_t>(mtot, 1) * 4));
         k_pd_msgs<true><<<(unsigned)tiles_v, PD_THREADS, 0, st>>>(V, (uint32_t)nmsg, B.voff, B.tbnd, B.sinfo, hd,
-                                                                  B.deg, n, nullptr, B.tcnt, mheads, B.moff, err);
+                                                                  B.deg, n, B.vkeep, nullptr, B.tcnt, mheads, B.moff,
+                                                                  err);
         fill_u64(B.moff + nmsg, 1, mtot, st);
         if (unsorted && mtot) {
             uint64_t* keys = static_cast<uint64_t*>(cv.take(mtot * 8));
