@@ -109,8 +109,17 @@ void scan_u64(const uint64_t* in, uint64_t* out, uint64_t len, uint64_t* tmp, hi
 // ---------------------------------------------------------------------------
 // degrees + offset validation
 // ---------------------------------------------------------------------------
+// One-byte degree class, monotone in the degree: the degree itself below
+// 128, above it 4 classes per octave (the two bits after the leading one).
+// Equal classes below 128 mean equal degrees.
+__device__ __forceinline__ uint32_t deg_class(uint32_t d) {
+    if (d < 128) return d;
+    const uint32_t lg = 31 - __clz(d);  // >= 7
+    return 128 + 4 * (lg - 7) + ((d >> (lg - 2)) & 3);
+}
+
 __global__ void k_degree(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, uint32_t* __restrict__ deg,
-                         uint32_t* __restrict__ err) {
+                         uint8_t* __restrict__ dcls, uint32_t* __restrict__ err) {
     uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v == 0 && (off[0] != 0 || off[n] != nnz)) atomicOr(err, KT_ERR_OFFSETS);
     if (v >= n) return;
@@ -118,9 +127,11 @@ __global__ void k_degree(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ 
     if (b < a || b - a > 0xffffffffull) {
         atomicOr(err, KT_ERR_OFFSETS);
         deg[v] = 0;
+        dcls[v] = 0;
         return;
     }
     deg[v] = (uint32_t)(b - a);
+    dcls[v] = (uint8_t)deg_class((uint32_t)(b - a));
 }
 
 // ---------------------------------------------------------------------------
@@ -163,7 +174,8 @@ __global__ void k_tile_bounds(uint32_t n, uint64_t nnz, uint64_t tile, const uin
 
 __global__ void __launch_bounds__(OR_THREADS)
     k_orient_keep(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
-                  const uint32_t* __restrict__ deg, const uint32_t* __restrict__ tbnd, uint64_t* __restrict__ keep_bits,
+                  const uint32_t* __restrict__ deg, const uint8_t* __restrict__ dcls, const uint32_t* __restrict__ tbnd,
+                  uint64_t* __restrict__ keep_bits,
                   uint64_t* __restrict__ dplus, uint64_t* __restrict__ tile_cnt, uint32_t* __restrict__ err) {
     __shared__ uint32_t marks[OR_TILE];
     __shared__ uint32_t rowcnt[OR_TILE];
@@ -183,9 +195,10 @@ __global__ void __launch_bounds__(OR_THREADS)
     // come from the next lane (the last lane loads them), and kept slots are
     // counted per run of equal rows with one LDS atomic from the run's last lane.
     const unsigned lane = lane_id();
-    // all loads first (16 heads, then 32 degree gathers), so that a thread
-    // has every item's memory traffic in flight at once
-    uint32_t hu[OR_ITEMS], du[OR_ITEMS], dv[OR_ITEMS];
+    // all loads first (16 heads, then 32 one-byte degree-class gathers from
+    // an n-byte table that the L2 holds far better than the 4n-byte degrees),
+    // so that a thread has every item's memory traffic in flight at once
+    uint32_t hu[OR_ITEMS], cu[OR_ITEMS], cv[OR_ITEMS];
 #pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
         const uint64_t sl = tile0 + k * OR_THREADS + tid;
@@ -195,8 +208,8 @@ __global__ void __launch_bounds__(OR_THREADS)
     for (int k = 0; k < OR_ITEMS; ++k) {
         const unsigned i = k * OR_THREADS + tid;
         const bool in = tile0 + i < tend;
-        du[k] = (in && hu[k] < n) ? deg[hu[k]] : 0u;
-        dv[k] = in ? deg[lo + marks[i]] : 0u;
+        cu[k] = (in && hu[k] < n) ? dcls[hu[k]] : 0u;
+        cv[k] = in ? dcls[lo + marks[i]] : 0u;
     }
 #pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
@@ -212,7 +225,15 @@ __global__ void __launch_bounds__(OR_THREADS)
                 flags |= KT_ERR_HEAD_RANGE;
             } else {
                 if (u == v) flags |= KT_ERR_SELF_LOOP;
-                keep = keep_edge(v, dv[k], u, du[k]);
+                // classes decide unless they are equal above 128 (then the
+                // exact degrees, a rare gather); equal classes below 128 are
+                // equal degrees
+                if (cv[k] != cu[k])
+                    keep = cv[k] < cu[k];
+                else if (cv[k] < 128)
+                    keep = v < u;
+                else
+                    keep = keep_edge(v, deg[v], u, deg[u]);
             }
         }
         // next slot in the same row? (shuffles outside any branch: every lane active)
@@ -445,6 +466,7 @@ struct OrientWs {
     uint64_t* dplus;     // n + 1
     uint32_t* tbnd;      // 2 * tiles: rows of each tile's first and last slot
     uint64_t* stmp;
+    uint8_t* dcls;       // n: degree classes
 };
 
 static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
@@ -456,13 +478,15 @@ static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
     w.dplus = w.bits + (nnz + 63) / 64 + 1;
     w.tbnd = reinterpret_cast<uint32_t*>(w.dplus + (uint64_t)n + 1);
     w.stmp = w.dplus + (uint64_t)n + 1 + w.tiles;
+    w.dcls = reinterpret_cast<uint8_t*>(
+        w.stmp + std::max(scan_tmp_words(w.tiles + 1), scan_tmp_words((uint64_t)n + 1)));
     return w;
 }
 
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz) {
     uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
     return 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 + tiles +
-           std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1));
+           std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1)) + ((uint64_t)n + 7) / 8;
 }
 
 // Rows of the first and last slot of every `tile`-slot tile of a CSR whose
@@ -479,12 +503,12 @@ void tile_bounds_launch(uint32_t n, uint64_t nnz, uint64_t tile, const uint64_t*
 void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,
                          uint64_t* ooff, uint64_t* ws, uint32_t* err, hipStream_t st) {
     OrientWs w = orient_ws(n, nnz, ws);
-    k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, err);
+    k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, w.dcls, err);
     fill_u64(w.dplus, (uint64_t)n + 1, 0, st);
     if (w.tiles > 0) {
         fill_u32(w.tbnd, 2 * w.tiles, 0, st);
         k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, w.tbnd);
-        k_orient_keep<<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.tbnd, w.bits, w.dplus,
+        k_orient_keep<<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.dcls, w.tbnd, w.bits, w.dplus,
                                                                w.tile_cnt, err);
         scan_u64(w.tile_cnt, w.tile_off, w.tiles, w.stmp, st);
     } else {
