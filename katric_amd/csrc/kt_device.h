@@ -42,6 +42,24 @@ __device__ __forceinline__ T wave_sum(T x) {
     return x;
 }
 
+// One-byte degree class, monotone in the degree: the degree itself below
+// 128, above it 4 classes per octave (the two bits after the leading one).
+// Equal classes below 128 mean equal degrees.
+__device__ __forceinline__ uint32_t deg_class(uint32_t d) {
+    if (d < 128) return d;
+    const uint32_t lg = 31 - __clz(d);  // >= 7
+    return 128 + 4 * (lg - 7) + ((d >> (lg - 2)) & 3);
+}
+
+// (deg a, a) < (deg b, b) from one-byte degree classes, reading the exact
+// degrees only when the classes are equal above 127.
+__device__ __forceinline__ bool class_before(uint32_t ca, uint32_t a, uint32_t cb, uint32_t b, const uint32_t* deg) {
+    if (ca != cb) return ca < cb;
+    if (ca < 128) return a < b;
+    const uint32_t da = deg[a], db = deg[b];
+    return da < db || (da == db && a < b);
+}
+
 __device__ __forceinline__ unsigned long long atomic_add_u64(uint64_t* p, uint64_t v) {
     return atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }

@@ -109,15 +109,6 @@ void scan_u64(const uint64_t* in, uint64_t* out, uint64_t len, uint64_t* tmp, hi
 // ---------------------------------------------------------------------------
 // degrees + offset validation
 // ---------------------------------------------------------------------------
-// One-byte degree class, monotone in the degree: the degree itself below
-// 128, above it 4 classes per octave (the two bits after the leading one).
-// Equal classes below 128 mean equal degrees.
-__device__ __forceinline__ uint32_t deg_class(uint32_t d) {
-    if (d < 128) return d;
-    const uint32_t lg = 31 - __clz(d);  // >= 7
-    return 128 + 4 * (lg - 7) + ((d >> (lg - 2)) & 3);
-}
-
 __global__ void k_degree(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, uint32_t* __restrict__ deg,
                          uint8_t* __restrict__ dcls, uint32_t* __restrict__ err) {
     uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -225,15 +216,9 @@ __global__ void __launch_bounds__(OR_THREADS)
                 flags |= KT_ERR_HEAD_RANGE;
             } else {
                 if (u == v) flags |= KT_ERR_SELF_LOOP;
-                // classes decide unless they are equal above 128 (then the
-                // exact degrees, a rare gather); equal classes below 128 are
-                // equal degrees
-                if (cv[k] != cu[k])
-                    keep = cv[k] < cu[k];
-                else if (cv[k] < 128)
-                    keep = v < u;
-                else
-                    keep = keep_edge(v, deg[v], u, deg[u]);
+                // classes decide unless they are equal above 127 (then the
+                // exact degrees, a rare gather)
+                keep = class_before(cv[k], v, cu[k], u, deg);
             }
         }
         // next slot in the same row? (shuffles outside any branch: every lane active)
