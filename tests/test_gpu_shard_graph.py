@@ -146,3 +146,11 @@ def test_rejects_malformed_input(ctx):
         S.count_shard_graph(ctx, K.Graph(g.n, off, g.heads.copy()), 4, 0)
     # the context still counts after the errors
     assert S.count_shard_graph(ctx, g, 4, 0, per_vertex=False)[:2] == O.rank_sim(g, 4, 0)[:2]
+
+
+@pytest.mark.parametrize("algo", [S.CETRIC, S.DITRIC])
+def test_dense_messages(ctx, algo):
+    # average degree 800 over 8 ranks: most contracted-phase messages carry
+    # 256-600 ghost entries, so workgroup teams (receivers shared a few per
+    # grab) count them, not only single waves
+    check_all_ranks(ctx, K.gen_gnm(4000, 1600000, seed=11), 8, algo, device=True)
