@@ -276,32 +276,39 @@ __global__ void __launch_bounds__(OR_THREADS)
 }
 
 // Copies the kept heads in slot order: output position = tile base + rank of
-// the slot among the tile's kept slots.
+// the slot among the tile's kept slots.  Every wave loads its 16 bitmask
+// words first; one barrier publishes the per-(round, wave) counts, from which
+// each wave derives all its output positions.
 __global__ void __launch_bounds__(OR_THREADS)
     k_orient_emit(uint64_t nnz, const uint32_t* __restrict__ heads, const uint64_t* __restrict__ keep_bits,
                   const uint64_t* __restrict__ tile_off, uint32_t* __restrict__ oheads) {
-    __shared__ uint32_t wcnt[OR_THREADS / KT_WAVE];
+    constexpr int NW = OR_THREADS / KT_WAVE;
+    __shared__ uint32_t wcnt[OR_ITEMS][NW];
     const uint64_t tile0 = (uint64_t)blockIdx.x * OR_TILE;
-    const unsigned tid = threadIdx.x, wave = tid / KT_WAVE;
-    uint64_t base = tile_off[blockIdx.x];
+    const unsigned tid = threadIdx.x, wave = tid / KT_WAVE, lane = lane_id();
+    uint64_t bits[OR_ITEMS];
+#pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
         const uint64_t w0 = tile0 + k * OR_THREADS + wave * KT_WAVE;
-        const uint64_t bits = (w0 < nnz) ? keep_bits[w0 / 64] : 0;
-        if (lane_id() == 0) wcnt[wave] = (uint32_t)__popcll(bits);
-        __syncthreads();
+        bits[k] = (w0 < nnz) ? keep_bits[w0 / 64] : 0;
+        if (lane == 0) wcnt[k][wave] = (uint32_t)__popcll(bits[k]);
+    }
+    __syncthreads();
+    uint64_t base = tile_off[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < OR_ITEMS; ++k) {
         uint32_t before = 0, round = 0;
 #pragma unroll
-        for (unsigned w = 0; w < OR_THREADS / KT_WAVE; ++w) {
-            const uint32_t c = wcnt[w];
+        for (unsigned w = 0; w < NW; ++w) {
+            const uint32_t c = wcnt[k][w];
             before += (w < wave) ? c : 0;
             round += c;
         }
-        if ((bits >> lane_id()) & 1) {
-            const uint64_t pos = base + before + __popcll(bits & lanemask_lt());
-            oheads[pos] = heads[w0 + lane_id()];
+        if ((bits[k] >> lane) & 1) {
+            const uint64_t w0 = tile0 + k * OR_THREADS + wave * KT_WAVE;
+            oheads[base + before + __popcll(bits[k] & lanemask_lt())] = heads[w0 + lane];
         }
         base += round;
-        __syncthreads();
     }
 }
 
