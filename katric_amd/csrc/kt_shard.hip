@@ -230,36 +230,35 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t x) {
     return x;
 }
 
-// Inclusive prefix sum over the lanes of this lane's group (every lane of the
-// group active).
-__device__ __forceinline__ uint32_t group_incl_sum(uint32_t x) {
-    const unsigned gl = lane_id() & (MSG_G - 1);
-#pragma unroll
-    for (unsigned d = 1; d < MSG_G; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d, MSG_G);
-        if (gl >= d) x += y;
-    }
-    return x;
-}
-
 // One step of a group over entries [p0, p0 + MSG_G * MSG_E) of a message
-// ending at e: this lane's entries p0 + MSG_E * gl + k, their combined
-// indices (0xffffffff past the end or unknown to the rank).
+// ending at e: this lane's entries p0 + gl + MSG_G * k (round k covers
+// MSG_G consecutive entries, one per lane, so every load and store of a
+// round is one contiguous run) and their combined indices (0xffffffff past
+// the end or unknown to the rank).
 struct MsgStep {
     uint32_t x[MSG_E], c[MSG_E];
-    __device__ __forceinline__ void load(const ShardMap& M, const uint32_t* __restrict__ mheads, uint64_t p,
-                                         uint64_t e) {
+    __device__ __forceinline__ static uint64_t at(uint64_t p0, unsigned gl, unsigned k) {
+        return p0 + gl + MSG_G * k;
+    }
+    __device__ __forceinline__ void load(const ShardMap& M, const uint32_t* __restrict__ mheads, uint64_t p0,
+                                         unsigned gl, uint64_t e) {
 #pragma unroll
-        for (unsigned k = 0; k < MSG_E; ++k) x[k] = (p + k < e) ? mheads[p + k] : 0u;
+        for (unsigned k = 0; k < MSG_E; ++k) x[k] = (at(p0, gl, k) < e) ? mheads[at(p0, gl, k)] : 0u;
 #pragma unroll
-        for (unsigned k = 0; k < MSG_E; ++k) c[k] = (p + k < e) ? M.map(x[k]) : 0xffffffffu;
+        for (unsigned k = 0; k < MSG_E; ++k) c[k] = (at(p0, gl, k) < e) ? M.map(x[k]) : 0xffffffffu;
     }
     // the combined indices stored by a previous pass at the entries' positions
-    __device__ __forceinline__ void load_mapped(const uint32_t* __restrict__ mapped, uint64_t p, uint64_t e) {
+    __device__ __forceinline__ void load_mapped(const uint32_t* __restrict__ mapped, uint64_t p0, unsigned gl,
+                                                uint64_t e) {
 #pragma unroll
-        for (unsigned k = 0; k < MSG_E; ++k) c[k] = (p + k < e) ? mapped[p + k] : 0xffffffffu;
+        for (unsigned k = 0; k < MSG_E; ++k) c[k] = (at(p0, gl, k) < e) ? mapped[at(p0, gl, k)] : 0xffffffffu;
     }
 };
+
+// This group's bits of a wave ballot (lane gl of the group = bit gl).
+__device__ __forceinline__ uint32_t group_bits(uint64_t ballot) {
+    return (uint32_t)(ballot >> (lane_id() & ~(MSG_G - 1))) & ((1u << MSG_G) - 1);
+}
 
 // Per message: owned / ghost entry counts; sender -> combined index.  The
 // combined index of every entry is stored at the entry's position in `mapped`
@@ -278,16 +277,22 @@ __global__ void k_msg_count(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint
         uint32_t own = 0, gh = 0;  // this lane's share
         if (!bad) {
             for (uint64_t p0 = b; p0 < e; p0 += MSG_G * MSG_E) {  // uniform over the group
-                const uint64_t p = p0 + MSG_E * gl;
                 MsgStep s;
-                s.load(M, mheads, p, e);
-                const uint32_t xn = (p + MSG_E < e) ? mheads[p + MSG_E] : 0u;  // the next lane's first entry
+                s.load(M, mheads, p0, gl, e);
+                // the entry after this step's last one (read by the group's last lane)
+                const uint64_t pn = p0 + MSG_G * MSG_E;
+                const uint32_t xn = (gl == MSG_G - 1 && pn < e) ? mheads[pn] : 0u;
 #pragma unroll
                 for (unsigned k = 0; k < MSG_E; ++k) {
-                    if (p + k >= e) continue;
-                    mapped[p + k] = s.c[k];
-                    const uint32_t nx = (k + 1 < MSG_E) ? s.x[k + 1] : xn;
-                    if (p + k + 1 < e && nx <= s.x[k]) flags |= KT_ERR_MESSAGE;
+                    // successor of entry (k, gl): lane gl + 1 of round k, or lane 0
+                    // of round k + 1 (shuffles outside any branch: the group is whole)
+                    const uint32_t up = __shfl_down(s.x[k], 1, MSG_G);
+                    const uint32_t first = __shfl(s.x[k + 1 < MSG_E ? k + 1 : k], 0, MSG_G);
+                    const uint32_t nx = gl + 1 < MSG_G ? up : (k + 1 < MSG_E ? first : xn);
+                    const uint64_t q = MsgStep::at(p0, gl, k);
+                    if (q >= e) continue;
+                    mapped[q] = s.c[k];
+                    if (q + 1 < e && nx <= s.x[k]) flags |= KT_ERR_MESSAGE;
                     own += s.c[k] < M.nl ? 1u : 0u;
                     gh += (s.c[k] != 0xffffffffu && s.c[k] >= M.nl) ? 1u : 0u;
                 }
@@ -314,33 +319,25 @@ __global__ void k_msg_write(uint32_t nl, uint32_t nmsg, uint64_t mtot, const uin
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t groups = (uint64_t)gridDim.x * blockDim.x / MSG_G;
     const unsigned gl = lane_id() & (MSG_G - 1);
+    const uint32_t below = (1u << gl) - 1;  // this lane's predecessors in the group
     for (uint64_t i = t / MSG_G; i < nmsg; i += groups) {
         const uint64_t b = moff[i], e = moff[i + 1];
         uint64_t po = coff_msg[i], pg = coff_msg[i] + nown[i];
         if (gl == 0) split[i] = pg;
-        if (e < b || e > mtot) continue;  // flagged by k_msg_count; nothing was counted for it
+        if (e < b || e > mtot) continue;  // flagged by k_msg_count; nothing was counted for it (uniform per group)
         for (uint64_t p0 = b; p0 < e; p0 += MSG_G * MSG_E) {
-            const uint64_t p = p0 + MSG_E * gl;
             MsgStep s;
-            s.load_mapped(mapped, p, e);
-            uint32_t o = 0, h = 0;
+            s.load_mapped(mapped, p0, gl, e);
 #pragma unroll
             for (unsigned k = 0; k < MSG_E; ++k) {
-                o += s.c[k] < nl ? 1u : 0u;
-                h += (s.c[k] != 0xffffffffu && s.c[k] >= nl) ? 1u : 0u;
+                // round k holds consecutive entries in lane order
+                const bool o = s.c[k] < nl, h = s.c[k] != 0xffffffffu && s.c[k] >= nl;
+                const uint32_t ob = group_bits(__ballot(o)), hb = group_bits(__ballot(h));
+                if (o) out[po + __popc(ob & below)] = s.c[k];
+                if (h) out[pg + __popc(hb & below)] = s.c[k];
+                po += __popc(ob);
+                pg += __popc(hb);
             }
-            const uint32_t oi = group_incl_sum(o), hi = group_incl_sum(h);
-            const uint32_t ot = __shfl(oi, MSG_G - 1, MSG_G), ht = __shfl(hi, MSG_G - 1, MSG_G);
-            uint64_t qo = po + (oi - o), qg = pg + (hi - h);
-#pragma unroll
-            for (unsigned k = 0; k < MSG_E; ++k) {
-                if (s.c[k] < nl)
-                    out[qo++] = s.c[k];
-                else if (s.c[k] != 0xffffffffu)
-                    out[qg++] = s.c[k];
-            }
-            po += ot;
-            pg += ht;
         }
     }
 }
