@@ -477,10 +477,11 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     trace("intersection: X=%u pv=%p", n, (void*)pv);
     // one graph's wave tails vary in work: smaller grabs even out the end of
     // the launch (RMAT-22: k_tc_main 21.0 ms with 32 tails per grab, 20.6
-    // with 16, 20.4 with 8; the shard and batch views, whose tails are many
-    // and tiny, are fastest with 32)
+    // with 16, 20.4-20.6 with 8; G(n,m) 2^16: 0.205 ms with 16, 0.187 with 8;
+    // the shard and batch views, whose tails are many and tiny, are fastest
+    // with 32)
     TcParams pp = c->prm;
-    pp.wave_grab = 16;
+    pp.wave_grab = 8;
     rc = c->hip_check(tc_launch(view, out, pp, B.ws_tc, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]),
                       "intersection launch");
     if (rc) return rc;
