@@ -56,8 +56,10 @@ def parse():
     ap.add_argument("--mode", choices=["plain", "batch", "serve", "shard", "copy"], default="plain")
     ap.add_argument("--requests", type=int, default=20000, help="serve mode: requests in the Poisson stream")
     ap.add_argument("--rate", type=float, default=0.0, help="serve mode: arrivals/s (0: 70%% of saturation)")
-    ap.add_argument("--pool", type=int, default=256, help="serve mode: distinct graphs the requests draw from")
-    ap.add_argument("--baseline-requests", type=int, default=1500, help="serve mode: one-graph-per-launch prefix")
+    ap.add_argument("--small-mix", action="store_true",
+                    help="serve mode: the config-4 graph mix (n in [64, 65536]) instead of config 5")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="plain mode: only the RMAT-22 line (no unsorted, batch and serve legs)")
     ap.add_argument("--max-batch-edges", type=int, default=0,
                     help="serve mode: edges per device batch (0: library default)")
     ap.add_argument("--copy-mib", type=int, default=1024, help="copy mode: bytes copied per launch, MiB")
@@ -313,15 +315,33 @@ def _sleep_until(t_abs):
         time.sleep(d - 0.0002 if d > 0.0005 else 0)
 
 
-def run_serve(args):
+# Request mixes of the served stream: config 5 (n log-uniform in
+# [2^10, 2^20], average degree U[4, 32]) and the small-graph mix of config 4
+# (n in [64, 65536], average degree U[4, 64]), where batching pays most.
+SERVE_MIXES = {
+    "config5": dict(n_min=1 << 10, n_max=1 << 20, deg_min=4.0, deg_max=32.0, pool=256, baseline=1500,
+                    saturation=4000),
+    "small": dict(n_min=64, n_max=65536, deg_min=4.0, deg_max=64.0, pool=2048, baseline=4000, saturation=20000),
+}
+
+
+def serve_leg(args, ctx, mix, requests):
+    """A Poisson stream of `requests` count requests of the given mix through
+    the continuous-batching server (kt_serve_*), against one kt_count call per
+    request on the same GPU.  Requests draw from a seeded pool of distinct
+    graphs in page-locked memory; every request uploads its CSR again (the
+    server keeps no results), so reuse of a pool graph saves no work."""
     import threading
 
     import katric_amd as K
     from katric_amd import batch as KB
     from katric_amd import serve as KS
+    from katric_amd import tune as T
 
+    mx = SERVE_MIXES[mix]
     t0 = time.time()
-    pool_b = KB.gen_batch(args.pool, seed=args.seed, n_min=1 << 10, n_max=1 << 20, deg_min=4.0, deg_max=32.0)
+    pool_b = KB.gen_batch(mx["pool"], seed=args.seed + (0 if mix == "config5" else 17), n_min=mx["n_min"],
+                          n_max=mx["n_max"], deg_min=mx["deg_min"], deg_max=mx["deg_max"])
     pool, owners = [], []
     for g in range(pool_b.num_graphs):
         view, own = KS.pinned_graph(pool_b.graph(g))
@@ -329,18 +349,22 @@ def run_serve(args):
         owners += own
     del pool_b
     pool_edges = np.array([g.m for g in pool], np.int64)
-    log(f"[bench] serve pool: {len(pool)} graphs, mean m {pool_edges.mean():.0f}, gen {time.time() - t0:.1f}s")
-    rng = np.random.default_rng(args.seed)
-    R = args.requests
+    log(f"[bench] serve {mix}: pool {len(pool)} graphs, mean m {pool_edges.mean():.0f}, gen {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(args.seed + 3)
+    R = requests
     which = rng.integers(0, len(pool), size=R)
-    from katric_amd import tune as T
-    ctx = K.Context(0)
-    # device batches of the server mix pool graphs: the class of a 64-graph sample
     sample = [pool[i] for i in range(min(64, len(pool)))]
     sample_off = np.concatenate([[0]] + [np.diff(gr.offsets.astype(np.int64)) for gr in sample]).cumsum()
     knobs = apply_params(ctx, args, T.batch_key(len(sample), sample_off))
-    # reference totals, one kt_count per pool graph (also warms the plain path)
+    # reference totals: one kt_count per pool graph (this also warms the
+    # one-graph path); a sample of them against the oracle
     want = np.array([ctx.count(g, per_vertex=False)[0] for g in pool], np.uint64)
+    checked = 0
+    if not args.no_check:
+        from oracle import oracle as O
+        for i in range(0, len(pool), max(1, len(pool) // 16)):
+            assert O.run_shmem(pool[i], threads=args.cpu_threads, per_vertex=False)[0] == want[i]
+            checked += 1
 
     def run_stream(srv, n, arrivals):
         tickets = [0] * n
@@ -366,10 +390,11 @@ def run_serve(args):
             assert r.status == 0 and r.total == want[which[i]], "served count differs from the one-graph count"
         return res, wall
 
-    # saturation throughput: everything submitted at once
     srv = KS.Server(ctx, max_batch_edges=args.max_batch_edges)
-    n_sat = min(R, 4000)
-    res, wall = run_stream(srv, n_sat, None)
+    # saturation: every request submitted at once
+    n_sat = min(R, mx["saturation"])
+    run_stream(srv, min(n_sat, 500), None)  # warm-up (captures, arena growth)
+    _, wall = run_stream(srv, n_sat, None)
     sat_rate = n_sat / wall
     rate = args.rate if args.rate > 0 else 0.7 * sat_rate
     arrivals = _poisson_times(R, rate, np.random.default_rng(args.seed + 1))
@@ -379,86 +404,85 @@ def run_serve(args):
     lat = [r.latency_ms for r in res]
     p50, p99 = _percentiles(lat)
     edges = int(pool_edges[which[:R]].sum())
-    served_gps = R / wall
-    # one kt_count per request on the same stream (prefix), same GPU
-    nb = min(R, args.baseline_requests)
-    b_lat = []
-    base = time.perf_counter()
-    for i in range(nb):
-        _sleep_until(base + arrivals[i])
-        t_arr = base + arrivals[i]
-        tot, _ = ctx.count(pool[which[i]], per_vertex=False)
-        assert tot == want[which[i]]
-        b_lat.append((time.perf_counter() - t_arr) * 1e3)
-    b_wall = time.perf_counter() - base
-    # saturated one-graph-per-launch throughput on the same prefix
+    # one kt_count per request, same GPU: saturated, then on the same arrivals
+    nb = min(R, mx["baseline"])
     ts = time.perf_counter()
     for i in range(nb):
         ctx.count(pool[which[i]], per_vertex=False)
     b_sat = nb / (time.perf_counter() - ts)
+    b_lat = []
+    base = time.perf_counter()
+    for i in range(nb):
+        _sleep_until(base + arrivals[i])
+        tot, _ = ctx.count(pool[which[i]], per_vertex=False)
+        assert tot == want[which[i]]
+        b_lat.append((time.perf_counter() - (base + arrivals[i])) * 1e3)
     b50, b99 = _percentiles(b_lat)
+    for o in owners:
+        o.free()
+    return {
+        "mix": mix,
+        "graphs_per_s": R / wall, "edges_per_s": edges / wall, "requests": R, "edges": edges,
+        "arrival_rate_per_s": rate, "latency_p50_ms": p50, "latency_p99_ms": p99,
+        "saturation_graphs_per_s": sat_rate,
+        "one_graph_per_launch_graphs_per_s": b_sat,
+        "speedup_saturation_vs_one_graph_per_launch": sat_rate / b_sat,
+        "one_graph_per_launch_latency_p50_ms_same_arrivals": b50,
+        "one_graph_per_launch_latency_p99_ms_same_arrivals": b99,
+        "one_graph_per_launch_requests": nb,
+        "batches": stats["batches"], "mean_batch_graphs": stats["graphs"] / max(stats["batches"], 1),
+        "pool_graphs": len(pool), "pool_mean_edges": float(pool_edges.mean()),
+        "params": knobs,
+        "checked": f"every served total equals the one-graph count; {checked} pool graphs equal the oracle",
+        "pcie": "CSR uploaded per request from page-locked memory (PCIe inclusive)",
+    }
+
+
+def run_serve(args):
+    import katric_amd as K
+
+    ctx = K.Context(0)
+    mix = "small" if args.small_mix else "config5"
+    leg = serve_leg(args, ctx, mix, args.requests)
     result = {
-        "metric": "graphs/s served (exact triangle counts, Poisson stream, continuous batching)",
-        "value": served_gps,
-        "unit": "graphs/s",
-        "n_gpus": 1,
-        "steps": R,
-        "warmup": n_sat,
-        "ms_per_step": wall * 1e3 / R,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32 ids / u64 counts",
-        "data": f"synthetic: requests drawn from a seeded pool of {len(pool)} G(n,m)/RMAT graphs "
-                f"(n log-uniform [2^10, 2^20], avg degree U[4,32]) in page-locked host memory",
-        "config": {
-            "params": knobs,
-            "workload": f"{R} requests, Poisson arrivals at {rate:.0f}/s, CSR uploaded per request (PCIe inclusive), "
-                        f"HBM-budget admission, continuous batching",
-            "arrival_rate_per_s": rate, "requests": R, "edges": edges, "edges_per_s": edges / wall,
-            "latency_p50_ms": p50, "latency_p99_ms": p99,
-            "saturation_graphs_per_s": sat_rate,
-            "batches": stats["batches"], "mean_batch_graphs": stats["graphs"] / max(stats["batches"], 1),
-            "one_graph_per_launch": {
-                "requests": nb, "saturation_graphs_per_s": b_sat,
-                "latency_p50_ms_same_stream": b50, "latency_p99_ms_same_stream": b99,
-                "wall_s_same_stream": b_wall,
-            },
-            "speedup_saturation_vs_one_graph_per_launch": sat_rate / b_sat,
-            "max_batch_edges": args.max_batch_edges or "library default (256 Mi)",
-        },
-        "cpu_baseline": None,
+        "metric": f"graphs/s served (exact triangle counts, Poisson stream, continuous batching, {mix} mix)",
+        "value": leg["graphs_per_s"], "unit": "graphs/s", "n_gpus": 1, "steps": leg["requests"], "warmup": 0,
+        "ms_per_step": 1e3 / leg["graphs_per_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 ids / u64 counts", "data": "synthetic (seeded pool of G(n,m)/RMAT graphs)",
+        "config": {"workload": f"served stream, {mix} mix", **leg}, "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)
 
 
-def run_batch(args):
+def batch_leg(args, ctx, graphs=8192):
+    """Config 4: `graphs` mixed graphs counted by the hipGraph-replayed batch
+    pipeline, against one kt_count launch per graph (CSR in HBM) on the same
+    GPU; a sample of the per-graph totals against the oracle."""
     import katric_amd as K
     from katric_amd import batch as KB
+    from katric_amd import tune as T
 
     t0 = time.time()
-    b = KB.gen_batch(args.graphs, seed=args.seed)
+    b = KB.gen_batch(graphs, seed=args.seed)
     G, N, E = b.num_graphs, b.n_total, b.nnz // 2
     log(f"[bench] batch: {G} graphs, {N} vertices, {E} edges, gen {time.time() - t0:.1f}s")
-    from katric_amd import tune as T
-    ctx = K.Context(0)
     knobs = apply_params(ctx, args, T.batch_key(G, b.offsets))
     db = KB.upload_batch(ctx, b)
     pv = ctx.alloc(max(N, 1) * 8)
-    # batched: the first call captures the pipeline, the rest replay it
     tot = None
-    for _ in range(args.warmup):
+    for _ in range(max(args.warmup, 1)):  # the first call captures the pipeline
         tot = KB.count_batch(ctx, db, per_vertex_out=pv)
     ctx.synchronize()
+    steps = max(args.steps // 2, 3)
     ts = time.perf_counter()
     step_ms = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         t1 = time.perf_counter()
         t2 = KB.count_batch(ctx, db, per_vertex_out=pv)  # returns after the totals are on the host
         step_ms.append((time.perf_counter() - t1) * 1e3)
         assert np.array_equal(t2, tot), "batch totals differ between replays"
     ctx.synchronize()
-    batch_s = (time.perf_counter() - ts) / args.steps
+    batch_s = (time.perf_counter() - ts) / steps
     replays = KB.replays(ctx)
     # one kt_count launch per graph on per-graph CSRs (local ids) in HBM
     vb, off, heads = b.vertex_base.astype(np.int64), b.offsets, b.heads
@@ -475,7 +499,7 @@ def run_batch(args):
     d_hd.upload(loc_heads)
     del loc_off, loc_heads
     single = np.zeros(G, np.uint64)
-    for g in range(min(G, 16)):  # warm the per-graph path (same knobs: tuned for these graphs)
+    for g in range(min(G, 16)):  # warm the per-graph path (same knobs)
         single[g] = ctx.count_device(int(vb[g + 1] - vb[g]), d_off.ptr + 8 * int(vb[g] + g),
                                      d_hd.ptr + 4 * int(off[vb[g]]), pv.ptr + 8 * int(vb[g]))
     ctx.synchronize()
@@ -489,39 +513,90 @@ def run_batch(args):
     single_s = time.perf_counter() - ts
     b50, b99 = _percentiles(step_ms)
     assert np.array_equal(single, tot), "batched totals differ from one-graph-per-launch totals"
+    checked = 0
+    if not args.no_check:
+        from oracle import oracle as O
+        for g in range(0, G, max(1, G // 64)):
+            assert O.run_shmem(b.graph(g), threads=args.cpu_threads, per_vertex=False)[0] == tot[g]
+            checked += 1
+    for buf in (d_off, d_hd, pv, db.offsets, db.heads):
+        buf.free()
+    return {
+        "graphs_per_s": G / batch_s, "edges_per_s": E / batch_s, "ms_per_batch": batch_s * 1e3,
+        "graphs": G, "vertices": N, "edges": E, "triangles": int(tot.sum()),
+        "one_graph_per_launch_graphs_per_s": G / single_s, "one_graph_per_launch_s": single_s,
+        "speedup_vs_one_graph_per_launch": single_s / batch_s,
+        # all graphs of the batch submitted at once: a graph's latency is its
+        # batch's wall time (batched) or the time until its own launch
+        # returned (one graph per launch, in submission order)
+        "latency_p50_ms": b50, "latency_p99_ms": b99,
+        "one_graph_per_launch_latency_p50_ms": float(np.percentile(done_ms, 50)),
+        "one_graph_per_launch_latency_p99_ms": float(np.percentile(done_ms, 99)),
+        "graph_replays": replays, "params": knobs,
+        "workload": f"{G} graphs, n log-uniform [64, 65536], avg degree U[4,64], G(n,m)/RMAT 50/50, "
+                    f"seed={args.seed}; disjoint-union CSR resident in HBM -> per-graph uint64 totals",
+        "checked": f"batched totals = one-graph-per-launch totals (all {G}); {checked} graphs equal the oracle",
+    }
+
+
+def run_batch(args):
+    import katric_amd as K
+
+    ctx = K.Context(0)
+    leg = batch_leg(args, ctx, args.graphs)
     result = {
-        "metric": "graphs/s (exact triangle counts, 8192 mixed G(n,m)/RMAT graphs, batched)",
-        "value": G / batch_s,
-        "unit": "graphs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": batch_s * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32 ids / u64 counts",
-        "data": "synthetic (seeded mixed batch generator)",
-        "config": {
-            "params": knobs,
-            "workload": f"{G} graphs, n log-uniform [64, 65536], avg degree U[4,64], G(n,m)/RMAT 50/50, "
-                        f"seed={args.seed}; disjoint-union CSR resident in HBM -> per-graph uint64 totals",
-            "graphs": G, "vertices": N, "edges": E, "triangles": int(tot.sum()),
-            "edges_per_s": E / batch_s,
-            "one_graph_per_launch_graphs_per_s": G / single_s,
-            "one_graph_per_launch_s": single_s,
-            "speedup_vs_one_graph_per_launch": single_s / batch_s,
-            # all graphs of the batch submitted at once: a graph's latency is
-            # its batch's wall time (batched) or the time until its own launch
-            # returned (one graph per launch, in submission order)
-            "latency_p50_ms": b50, "latency_p99_ms": b99,
-            "one_graph_per_launch_latency_p50_ms": float(np.percentile(done_ms, 50)),
-            "one_graph_per_launch_latency_p99_ms": float(np.percentile(done_ms, 99)),
-            "graph_replays": replays,
-        },
-        "cpu_baseline": None,
+        "metric": f"graphs/s (exact triangle counts, {leg['graphs']} mixed G(n,m)/RMAT graphs, batched)",
+        "value": leg["graphs_per_s"], "unit": "graphs/s", "n_gpus": 1, "steps": max(args.steps // 2, 3),
+        "warmup": args.warmup, "ms_per_step": leg["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32 ids / u64 counts", "data": "synthetic (seeded mixed batch generator)",
+        "config": leg, "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)
+
+
+def design_bytes(wedge_checks, m, n):
+    """HBM bytes the hash-probe intersection must move per launch: every
+    streamed T(u) once per probe (4 B per id; the reference's wedge_checks),
+    every S(x) once when its table is built (4m), the tail offsets and the
+    per-vertex counts (16n)."""
+    return 4 * wedge_checks + 4 * m + 16 * n
+
+
+def unsorted_leg(args, ctx, g, total):
+    """The same graph with every adjacency list in descending order, resident
+    in HBM: orientation finds the lists unsorted and sorts the oriented lists
+    with the device radix sort -- the path the reference always pays
+    (sort_neighborhoods, distributed_graph.h:577-593)."""
+    deg = np.diff(g.offsets.astype(np.int64))
+    rows = np.repeat(np.arange(g.n, dtype=np.int64), deg)
+    pos = g.offsets[rows].astype(np.int64) + g.offsets[rows + 1].astype(np.int64) - 1 - np.arange(g.nnz)
+    heads = np.empty_like(g.heads)
+    heads[pos] = g.heads
+    del rows, pos, deg
+    import katric_amd as K
+    dg = ctx.upload(K.Graph(g.n, g.offsets, heads))
+    del heads
+    d_pv = ctx.alloc(g.n * 8)
+    for _ in range(2):
+        assert ctx.count_device(dg, per_vertex_out=d_pv) == total
+    ctx.synchronize()
+    steps = max(args.steps // 4, 3)
+    prep, dev = [], []
+    ts = time.perf_counter()
+    for _ in range(steps):
+        assert ctx.count_device(dg, per_vertex_out=d_pv) == total, "unsorted input counts differ"
+        tm = ctx.timing()
+        prep.append(tm["prep_ms"])
+        dev.append(tm["total_ms"])
+    ctx.synchronize()
+    ms = (time.perf_counter() - ts) * 1e3 / steps
+    dg.free()
+    d_pv.free()
+    return {"ms_per_step": ms, "edges_per_s": g.m / (ms * 1e-3), "prep_ms": float(np.mean(prep)),
+            "device_ms": float(np.mean(dev)), "steps": steps,
+            "workload": "the same RMAT-22 CSR with every list in descending order, resident in HBM; orientation + "
+                        "device LSD radix sort of the oriented lists + intersection",
+            "checked": "total equals the sorted input's"}
 
 
 def run_plain(args):
@@ -545,30 +620,33 @@ def run_plain(args):
     ctx.synchronize()
     stats = ctx.stats()
     W = stats["intersection_size_local"]
+    WC = stats["wedge_checks"]
     kern_ms = []
-    step_ms = []
     ctx.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        ts = time.perf_counter()
         tt = ctx.count_device(dg, per_vertex_out=d_pv)
-        step_ms.append((time.perf_counter() - ts) * 1e3)
         assert tt == total, "non-deterministic total"
-        tm = ctx.timing()
-        kern_ms.append(tm["intersect_ms"])
+        kern_ms.append(ctx.timing()["intersect_ms"])
     ctx.synchronize()
     t_end = time.perf_counter()
     ms_per_step = (t_end - t_start) * 1e3 / args.steps
     tm = ctx.timing()
     pv_sum = int(d_pv.download(np.uint64, n).sum(dtype=np.uint64))
     assert pv_sum == 3 * total, "per-vertex counts inconsistent with the total"
+    dg.free()
+    d_pv.free()
 
-    # algorithmic bytes (unit = one oriented edge): B_count = 4W + 4m + 8(n+1) + 8n;
-    # B_pre = 16(n+1) + 12m (read symmetric CSR, write oriented CSR)
+    # roofline of k_tc_main on the bytes this design must move (unit = one
+    # oriented edge: 4 B per id of T(u) streamed -- wedge_checks ids in all --
+    # plus 4 B of S(x) per kept slot, plus 16 B per vertex); the reference's
+    # intersection_size_local figure, which charges both lists per edge
+    # (B_count = 4W + 4m + 8(n+1) + 8n), is reported beside it
+    b_design = design_bytes(WC, m, n)
     b_count = 4 * W + 4 * m + 8 * (n + 1) + 8 * n
     b_pre = 16 * (n + 1) + 12 * m
     k_ms = float(np.mean(kern_ms))
-    achieved = b_count / (k_ms * 1e-3) / 1e9
+    achieved = b_design / (k_ms * 1e-3) / 1e9
     pipeline_gbs = (b_pre + b_count) / (ms_per_step * 1e-3) / 1e9
 
     result = {
@@ -588,10 +666,9 @@ def run_plain(args):
             "params": knobs,
             "workload": f"RMAT-{args.scale} ef{args.edge_factor} a=0.57 b=0.19 c=0.19 seed={args.seed}, "
                         f"symmetric CSR resident in HBM -> total + per-vertex uint64 in HBM",
-            "n": n, "m": m, "triangles": total, "W": W, "csr_checksum": f"{checksum:016x}",
+            "n": n, "m": m, "triangles": total, "W": W, "wedge_checks": WC, "csr_checksum": f"{checksum:016x}",
             "device_ms": tm["total_ms"], "prep_ms": tm["prep_ms"], "count_ms": tm["count_ms"],
-            "pipeline_algorithmic_GBs": pipeline_gbs,
-            "pipeline_roofline_frac": pipeline_gbs / HBM_PEAK_GBS,
+            "pipeline_reference_bytes_GBs": pipeline_gbs,
         },
         "roofline": {
             "kernel": "k_tc_main (intersection)",
@@ -603,6 +680,10 @@ def run_plain(args):
             # measured under the autotuned knobs; other knobs move other bytes
             "traffic": None if args.params else MEASURED_TRAFFIC.get(("plain", args.scale, args.edge_factor, args.seed)),
             "kernel_ms": k_ms,
+            "bytes_per_launch": b_design,
+            "bytes": "4 * wedge_checks + 4 * m + 16 * n",
+            "algorithmic_effective_GBs": b_count / (k_ms * 1e-3) / 1e9,
+            "algorithmic_bytes": "4 * W + 4 * m + 8 * (n + 1) + 8 * n, W = intersection_size_local",
         },
         "cpu_baseline": None,
     }
@@ -627,20 +708,32 @@ def run_plain(args):
             "sample": f"full RMAT-{args.scale} workload, minimum of {len(runs)} runs (orient + sort + "
                       f"merge-intersection local phase, {thr} OpenMP threads), {cpu_s:.2f} s",
         }
-        # single-rank katric (CeTriC preset at P = 1, one thread) on a bounded
-        # sample: the same generator at 1/16 of the vertices
-        ss = max(args.scale - 4, 4)
-        gs = K.gen_rmat(ss, args.edge_factor, seed=args.seed)
+        # single-rank katric (CeTriC at P = 1 on one thread: with one rank
+        # there are no ghosts and no messages, so it is the one-thread local
+        # phase) on the same RMAT-22 graph: preprocessing in full, the local
+        # phase over every 16th tail (ids are randomly permuted, so the sample
+        # is representative); the full-workload time is prep + 16 x sample
+        stride = 16
         ts = time.perf_counter()
-        sl, s3, _, _ = O.rank_sim(gs, 1, 0, O.CETRIC, threads=1, per_vertex=False)
+        _, _, st1 = O.run_shmem_sample(g, stride=stride, per_vertex=False)
         one_s = time.perf_counter() - ts
-        if not args.no_check:
-            assert sl + s3 == ctx.count(gs, per_vertex=False)[0], "single-rank CPU count != GPU count on the sample"
+        est_s = st1["prep_s"] + stride * st1["local_s"]
         result["cpu_baseline_single_rank"] = {
-            "value": gs.m / one_s, "unit": "edges/s", "cores": 1, "kind": "port", **host_info(),
-            "sample": f"RMAT-{ss} ef{args.edge_factor} seed={args.seed} ({gs.m} edges), single-rank katric "
-                      f"(CeTriC preset, P = 1), one thread, {one_s:.2f} s",
+            "value": m / est_s, "unit": "edges/s", "cores": 1, "kind": "port", **host_info(),
+            "sample": f"RMAT-{args.scale} (the same graph): orientation + sort of all {m} edges "
+                      f"({st1['prep_s']:.2f} s) and the local phase over every {stride}th tail "
+                      f"({st1['local_s']:.2f} s, {st1['sample_edges']} of {m} oriented edges), one thread; "
+                      f"value = m / (prep + {stride} x local) = m / {est_s:.1f} s; measured {one_s:.1f} s",
         }
+    if not args.no_extra:
+        log("[bench] unsorted leg")
+        result["unsorted"] = unsorted_leg(args, ctx, g, total)
+        del g
+        log("[bench] batch leg")
+        result["batch"] = batch_leg(args, ctx)
+        log("[bench] serve legs")
+        result["serve"] = serve_leg(args, ctx, "config5", args.requests)
+        result["serve_small"] = serve_leg(args, ctx, "small", args.requests)
     print(json.dumps(result), flush=True)
 
 

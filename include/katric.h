@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define KATRIC_ABI_VERSION 1
+#define KATRIC_ABI_VERSION 2
 
 /* status codes */
 #define KT_OK 0
@@ -62,6 +62,7 @@ typedef struct {
     uint64_t oriented_edges; /* |E+| */
     uint64_t vertices;
     uint64_t input_slots;    /* entries of the symmetric input CSR */
+    uint64_t skipped_nodes;  /* tails skipped with pseudo2core (cetric_edge_iterator.h:239-242) */
 } kt_stats;
 
 /* Device times of the last call, from HIP events on the context stream. */
@@ -91,6 +92,17 @@ typedef struct {
     uint32_t grid_per_cu;   /* persistent 512-thread workgroups per compute unit (1..4 fill the CU) */
     uint32_t strategy;      /* KT_STRATEGY_* */
     uint32_t order;         /* KT_ORDER_* */
+    /* |N+(v)| above this: the tail's set is probed by binary search in global
+     * memory instead of an LDS hash (1..8192; 8192 = only sets beyond the
+     * LDS).  1 sends every tail to binary search: the device form of
+     * Config::intersection_method = binary_search (config.h:129,172). */
+    uint32_t bsearch_min_deg;
+    /* Config::pseudo2core (config.h:184): tails with |N+(v)| < 2 are skipped
+     * and counted in kt_stats.skipped_nodes (cetric_edge_iterator.h:239-242).
+     * Counts never change; the work counters do.  Shared-memory path only
+     * (kt_count, kt_list_triangles, kt_batch_count, serving): the shard
+     * entry points return KT_EINVAL while it is set. */
+    uint32_t pseudo2core;
 } kt_params;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes bounds the
@@ -155,6 +167,23 @@ int kt_list_triangles(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const ui
  * (:1057-1104) on one rank. */
 int kt_orient(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
               uint64_t* out_offsets, uint32_t* out_heads);
+
+/* Batched intersection of caller-given id lists: for pair i, the strictly
+ * increasing lists A_i = a_ids[a_offsets[i] .. a_offsets[i+1]) and
+ * B_i = b_ids[b_offsets[i] .. b_offsets[i+1]) (offsets: num_pairs + 1
+ * entries starting at 0).  counts[i] receives |A_i cap B_i| and *total_out
+ * their sum; if matches != NULL the common ids are written pair after pair,
+ * each pair's in ascending order (pair i's from the sum of counts[0..i) on),
+ * or KT_ERANGE is returned (counts and *total_out still set) when they exceed
+ * capacity (in ids).  A list that is not strictly increasing is KT_EINVAL.
+ * Inputs are host arrays unless KT_DEVICE_INPUT; counts / matches are host
+ * arrays unless KT_DEVICE_OUTPUT.  Replaces cetric::intersection
+ * (include/cetric/counters/intersection.h:40-79, its callback receiving every
+ * common element once, ascending as merge_intersection :81-96 emits them), as
+ * tests/intersection_test.cpp:35-173 calls it. */
+int kt_intersect(kt_ctx* ctx, uint64_t num_pairs, const uint64_t* a_offsets, const uint32_t* a_ids,
+                 const uint64_t* b_offsets, const uint32_t* b_ids, uint32_t flags, uint64_t* counts,
+                 uint32_t* matches, uint64_t capacity, uint64_t* total_out);
 
 /* Batched count of many independent graphs given as one disjoint-union CSR:
  * graph g owns the global ids [vertex_base[g], vertex_base[g+1]) (host array

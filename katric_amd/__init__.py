@@ -262,13 +262,13 @@ class Context:
         return {f: getattr(p, f) for f, _ in KtParams._fields_}
 
     def set_params(self, wave_max_deg=None, group_max_deg=None, grid_per_cu=None, strategy=None, order=None,
-                   **extra):
+                   bsearch_min_deg=None, pseudo2core=None, **extra):
         """Intersection knobs (see kt_params); unknown keys are rejected."""
         if extra:
             raise TypeError(f"unknown parameters: {sorted(extra)}")
         cur = self.params()
         new = dict(wave_max_deg=wave_max_deg, group_max_deg=group_max_deg, grid_per_cu=grid_per_cu,
-                   strategy=strategy, order=order)
+                   strategy=strategy, order=order, bsearch_min_deg=bsearch_min_deg, pseudo2core=pseudo2core)
         cur.update({k: int(v) for k, v in new.items() if v is not None})
         p = KtParams(**cur)
         self.check(lib().kt_set_params(self._h, C.byref(p)), "kt_set_params")
@@ -349,6 +349,35 @@ class Context:
                                      C.byref(total))
         self.check(rc, "kt_list_triangles")
         return out[: int(total.value)]
+
+    def intersect(self, pairs, with_matches: bool = True):
+        """Intersections of (a, b) pairs of strictly increasing id lists on the
+        device (kt_intersect).  Returns (counts, matches): counts[i] =
+        |a_i cap b_i|, matches a list of ascending uint32 arrays (None without
+        with_matches)."""
+        pairs = list(pairs)
+        P = len(pairs)
+        if P == 0:
+            return np.zeros(0, np.uint64), ([] if with_matches else None)
+        la = [np.ascontiguousarray(a, dtype=np.uint32) for a, _ in pairs]
+        lb = [np.ascontiguousarray(b, dtype=np.uint32) for _, b in pairs]
+        aoff = np.zeros(P + 1, np.uint64)
+        boff = np.zeros(P + 1, np.uint64)
+        aoff[1:] = np.cumsum([len(x) for x in la])
+        boff[1:] = np.cumsum([len(x) for x in lb])
+        a = np.concatenate(la) if aoff[-1] else np.zeros(1, np.uint32)
+        b = np.concatenate(lb) if boff[-1] else np.zeros(1, np.uint32)
+        counts = np.zeros(P, np.uint64)
+        total = C.c_uint64(0)
+        cap = int(min(aoff[-1], boff[-1]))
+        out = np.zeros(max(cap, 1), np.uint32) if with_matches else None
+        rc = lib().kt_intersect(self._h, P, _ptr(aoff), _ptr(a), _ptr(boff), _ptr(b), 0, _ptr(counts), _ptr(out),
+                                cap, C.byref(total))
+        self.check(rc, "kt_intersect")
+        if not with_matches:
+            return counts, None
+        pos = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        return counts, [out[pos[i]:pos[i + 1]].copy() for i in range(P)]
 
     def orient(self, g: Graph) -> Graph:
         """Oriented (out-only) CSR under the (degree, id) order, ascending lists."""

@@ -28,6 +28,7 @@ class KtStats(C.Structure):
         ("oriented_edges", u64),
         ("vertices", u64),
         ("input_slots", u64),
+        ("skipped_nodes", u64),
     ]
 
 
@@ -49,6 +50,8 @@ class KtParams(C.Structure):
         ("grid_per_cu", u32),
         ("strategy", u32),
         ("order", u32),
+        ("bsearch_min_deg", u32),
+        ("pseudo2core", u32),
     ]
 
 
@@ -76,6 +79,7 @@ SIGNATURES = {
     "kt_count": (i32, [vp, u32, vp, vp, u32, p_u64, vp]),
     "kt_list_triangles": (i32, [vp, u32, vp, vp, u32, vp, u64, p_u64]),
     "kt_orient": (i32, [vp, u32, vp, vp, u32, vp, vp]),
+    "kt_intersect": (i32, [vp, u64, vp, vp, vp, vp, u32, vp, vp, u64, p_u64]),
     # katric_graphio.h
     "kt_host_graph_free": (None, [C.POINTER(KtHostGraph)]),
     "kt_gen_gnm": (i32, [u32, u64, u64, C.POINTER(KtHostGraph)]),

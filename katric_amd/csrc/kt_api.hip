@@ -192,6 +192,7 @@ extern "C" int kt_create(kt_ctx** out, int device, uint64_t hbm_budget_bytes) {
         kt_destroy(c);
         return KT_EHIP;
     }
+    random_keys(c->sym_key);
     c->prm.wave_max_deg = 128;
     c->prm.group_max_deg = 2048;
     c->prm.team_chunk = 8;
@@ -238,7 +239,8 @@ extern "C" const char* kt_last_error(const kt_ctx* c) { return c ? c->err : "no 
 extern "C" int kt_set_params(kt_ctx* c, const kt_params* p) {
     if (!c || !p) return KT_EINVAL;
     if (p->wave_max_deg < 2 || p->wave_max_deg > 256 || p->group_max_deg < p->wave_max_deg ||
-        p->group_max_deg > 2048 || p->grid_per_cu < 1 || p->grid_per_cu > 32 || p->strategy > KT_STRATEGY_GROUPED2 || p->order > KT_ORDER_VERTEX)
+        p->group_max_deg > 2048 || p->grid_per_cu < 1 || p->grid_per_cu > 32 || p->strategy > KT_STRATEGY_GROUPED2 ||
+        p->order > KT_ORDER_VERTEX || p->bsearch_min_deg < 1 || p->bsearch_min_deg > 8192 || p->pseudo2core > 1)
         return KT_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
     c->prm.wave_max_deg = p->wave_max_deg;
@@ -246,6 +248,8 @@ extern "C" int kt_set_params(kt_ctx* c, const kt_params* p) {
     c->prm.grid_per_cu = p->grid_per_cu;
     c->prm.strategy = p->strategy;
     c->prm.order = p->order;
+    c->prm.bsearch_min_deg = p->bsearch_min_deg;
+    c->prm.pseudo2core = p->pseudo2core;
     return KT_OK;
 }
 
@@ -256,6 +260,8 @@ extern "C" int kt_get_params(const kt_ctx* c, kt_params* p) {
     p->grid_per_cu = c->prm.grid_per_cu;
     p->strategy = c->prm.strategy;
     p->order = c->prm.order;
+    p->bsearch_min_deg = c->prm.bsearch_min_deg;
+    p->pseudo2core = c->prm.pseudo2core;
     return KT_OK;
 }
 
@@ -349,7 +355,8 @@ int upload_and_count(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_o
     (void)hipEventRecord(c->ev[EV_H2D], st);
     trace("upload done, launching count pass");
     fill_u64(B.small, 64, 0, st);
-    orient_count_launch(n, nnz, B.off, B.heads, B.deg, B.ooff, B.ws_orient, reinterpret_cast<uint32_t*>(B.small),
+    OrientCheck chk{{c->sym_key[0], c->sym_key[1]}};
+    orient_count_launch(n, nnz, B.off, B.heads, B.deg, B.ooff, B.ws_orient, reinterpret_cast<uint32_t*>(B.small), chk,
                         st);
     rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st), "D2H flags");
     if (rc) return rc;
@@ -377,12 +384,14 @@ int run_orientation(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_of
     uint32_t flags = static_cast<uint32_t>(c->host_small[0]);
     uint64_t kept = c->host_small[1];
     trace("count pass: flags=0x%x kept=%llu", flags, (unsigned long long)kept);
-    if (flags & KT_ERR_MASK) {
+    if (flags & (KT_ERR_MASK & ~KT_ERR_ASYMMETRIC)) {
         return c->fail(KT_EINVAL, "malformed CSR:%s%s%s%s", (flags & KT_ERR_OFFSETS) ? " bad offsets" : "",
                        (flags & KT_ERR_HEAD_RANGE) ? " head id >= n" : "",
                        (flags & KT_ERR_SELF_LOOP) ? " self-loop" : "",
                        (flags & KT_ERR_DUPLICATE) ? " duplicate edge" : "");
     }
+    if (flags & KT_ERR_ASYMMETRIC)
+        return c->fail(KT_EINVAL, "CSR is not symmetric (some slot v -> u has no slot u -> v)");
     if (2 * kept != nnz) {
         return c->fail(KT_EINVAL, "CSR is not symmetric (%llu oriented edges for %llu slots)",
                        (unsigned long long)kept, (unsigned long long)nnz);
@@ -489,7 +498,7 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     uint64_t* tstats = tc_stats_ptr(B.ws_tc);
     rc = c->hip_check(hipMemcpyAsync(c->host_small + 1, totals, 8, hipMemcpyDeviceToHost, st), "D2H total");
     if (rc) return rc;
-    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tstats, 24, hipMemcpyDeviceToHost, st), "D2H stats");
     if (rc) return rc;
     if (want_pv && !pv_dev) {
         rc = c->hip_check(hipMemcpyAsync(per_vertex_out, pv, uint64_t(n) * 8, hipMemcpyDeviceToHost, st),
@@ -504,12 +513,13 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     c->stats.local_triangles = c->host_small[1];
     c->stats.wedge_checks = c->host_small[2];
     c->stats.intersection_size_local = c->host_small[3];
+    c->stats.skipped_nodes = c->host_small[4];
     c->timing.h2d_ms = c->elapsed(EV_START, EV_H2D);
-    c->timing.prep_ms = unsorted ? 0.f : c->elapsed(EV_H2D, EV_PREP);
+    c->timing.prep_ms = c->elapsed(EV_H2D, EV_PREP);
     c->timing.count_ms = c->elapsed(EV_PREP, EV_COUNT);
     c->timing.intersect_ms = c->elapsed(EV_TC0, EV_TC1);
     c->timing.d2h_ms = c->elapsed(EV_COUNT, EV_END);
-    c->timing.total_ms = unsorted ? 0.f : c->elapsed(EV_START, EV_END);
+    c->timing.total_ms = c->elapsed(EV_START, EV_END);
     return KT_OK;
 }
 
@@ -555,7 +565,7 @@ extern "C" int kt_list_triangles(kt_ctx* c, uint32_t n, const uint64_t* offsets,
                       "intersection launch");
     if (rc) return rc;
     scan_u64(tcnt, tcnt, n, stmp, st);
-    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tc_stats_ptr(B.ws_tc), 16, hipMemcpyDeviceToHost, st),
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tc_stats_ptr(B.ws_tc), 24, hipMemcpyDeviceToHost, st),
                       "D2H stats");
     if (rc) return rc;
     uint64_t total = 0;
@@ -565,6 +575,7 @@ extern "C" int kt_list_triangles(kt_ctx* c, uint32_t n, const uint64_t* offsets,
     c->stats.local_triangles = total;
     c->stats.wedge_checks = c->host_small[2];
     c->stats.intersection_size_local = c->host_small[3];
+    c->stats.skipped_nodes = c->host_small[4];
     if (total > capacity)
         return c->fail(KT_ERANGE, "%llu triangles do not fit a buffer of %llu", (unsigned long long)total,
                        (unsigned long long)capacity);
@@ -594,11 +605,11 @@ extern "C" int kt_list_triangles(kt_ctx* c, uint32_t n, const uint64_t* offsets,
     rc = c->hip_check(hipStreamSynchronize(st), "listing");
     if (rc) return rc;
     c->timing.h2d_ms = c->elapsed(EV_START, EV_H2D);
-    c->timing.prep_ms = unsorted ? 0.f : c->elapsed(EV_H2D, EV_PREP);
+    c->timing.prep_ms = c->elapsed(EV_H2D, EV_PREP);
     c->timing.count_ms = c->elapsed(EV_PREP, EV_COUNT);
     c->timing.intersect_ms = c->elapsed(EV_TC0, EV_TC1);
     c->timing.d2h_ms = c->elapsed(EV_COUNT, EV_END);
-    c->timing.total_ms = unsorted ? 0.f : c->elapsed(EV_START, EV_END);
+    c->timing.total_ms = c->elapsed(EV_START, EV_END);
     return KT_OK;
 }
 

@@ -75,7 +75,7 @@ __global__ void k_seg_partial(uint32_t G, uint64_t N, const uint64_t* __restrict
 __global__ void k_seg_final(uint32_t G, uint64_t* __restrict__ totals, uint32_t* __restrict__ err) {
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
         const uint64_t t = totals[g];
-        if (t % 3) atomicOr(err, 0x40u);
+        if (t % 3) atomicOr(err, KT_ERR_SEGMENT);
         totals[g] = t / 3;
     }
 }
@@ -115,12 +115,16 @@ static int bits_for(uint64_t x) {
 }
 
 // Enqueues the device pipeline on `st` (no host synchronisation): results land
-// in h_totals (per graph) and h_small ([0] flags, [1] kept edges, [2..3] stats).
+// in h_totals (per graph) and h_small ([0] flags, [1] kept edges, [2..4] stats).
 hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
-                    uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort) {
+                          uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort,
+                          const uint64_t sym_key[2]) {
     uint32_t* err = reinterpret_cast<uint32_t*>(B.small);
     fill_u64(B.small, 64, 0, st);
-    orient_count_launch((uint32_t)N, nnz, B.off, B.heads, B.deg, B.ooff, B.ws_orient, err, st);
+    // heads must stay inside their row's graph: a union with edges between
+    // graphs is not a batch of independent graphs
+    OrientCheck chk{{sym_key[0], sym_key[1]}, B.vbase, G};
+    orient_count_launch((uint32_t)N, nnz, B.off, B.heads, B.deg, B.ooff, B.ws_orient, err, chk, st);
     orient_write_launch((uint32_t)N, nnz, B.heads, B.oheads, B.ws_orient, st);
     if (sort) {
         const uint64_t m = nnz / 2;
@@ -143,7 +147,7 @@ hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, cons
     (void)hipMemcpyAsync(h_totals, B.totals, (uint64_t)G * 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(h_small, B.small, 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(h_small + 1, B.ooff + N, 8, hipMemcpyDeviceToHost, st);
-    (void)hipMemcpyAsync(h_small + 2, tc_stats_ptr(B.ws_tc), 16, hipMemcpyDeviceToHost, st);
+    (void)hipMemcpyAsync(h_small + 2, tc_stats_ptr(B.ws_tc), 24, hipMemcpyDeviceToHost, st);
     return hipGetLastError();
 }
 
@@ -238,7 +242,7 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
                 rc = c->hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "begin capture");
                 if (rc) return rc;
                 const hipError_t le = batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals,
-                                                     c->host_small, false);
+                                                     c->host_small, false, c->sym_key);
                 rc = c->hip_check(hipStreamEndCapture(st, &graph), "end capture");
                 if (!rc && le != hipSuccess) {
                     (void)hipGraphDestroy(graph);
@@ -267,7 +271,8 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
             S->replays++;
         } else {
             (void)hipEventRecord(c->ev[EV_PREP], st);
-            rc = c->hip_check(batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small, true),
+            rc = c->hip_check(batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small, true,
+                                              c->sym_key),
                               "batch pipeline launch");
             if (rc) return rc;
         }
@@ -293,6 +298,7 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
         c->stats.local_triangles = tot;
         c->stats.wedge_checks = c->host_small[2];
         c->stats.intersection_size_local = c->host_small[3];
+        c->stats.skipped_nodes = c->host_small[4];
         c->stats.oriented_edges = kept;
         c->stats.vertices = N;
         c->stats.input_slots = nnz;

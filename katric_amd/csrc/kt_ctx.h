@@ -46,6 +46,7 @@ struct kt_ctx {
     hipEvent_t ev[KT_NUM_EVENTS] = {};
     uint64_t* host_small = nullptr;  // pinned staging for flags / totals / stats
     kt::TcParams prm{};
+    uint64_t sym_key[2] = {0, 0};  // keys of the symmetry checksum (random per context)
     kt_stats stats{};
     kt_timing timing{};
     std::mutex mu;

@@ -60,6 +60,40 @@ __device__ __forceinline__ bool class_before(uint32_t ca, uint32_t a, uint32_t c
     return da < db || (da == db && a < b);
 }
 
+// 64-bit finaliser of MurmurHash3 (fmix64): a bijective mix in which every
+// input bit affects every output bit.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+// Signed contribution of the slot v -> u to a keyed checksum over unordered
+// pairs: +h({v,u}) when v < u, -h({v,u}) when v > u, so the two slots of an
+// undirected edge cancel.
+__device__ __forceinline__ uint64_t pair_term(uint32_t v, uint32_t u, uint64_t key) {
+    const uint64_t a = min(v, u), b = max(v, u);
+    const uint64_t h = mix64(((a << 32) | b) ^ key);
+    return v < u ? h : (0ull - h);
+}
+
+// Graph g in [glo, ghi] with vbase[g] <= v < vbase[g + 1] (the last g with
+// vbase[g] <= v, so that empty graphs sharing v's base are skipped).
+__device__ __forceinline__ uint32_t graph_of(const uint64_t* __restrict__ vbase, uint32_t glo, uint32_t ghi,
+                                             uint64_t v) {
+    while (glo < ghi) {
+        const uint32_t mid = glo + (ghi - glo + 1) / 2;
+        if (vbase[mid] <= v)
+            glo = mid;
+        else
+            ghi = mid - 1;
+    }
+    return glo;
+}
+
 __device__ __forceinline__ unsigned long long atomic_add_u64(uint64_t* p, uint64_t v) {
     return atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -356,11 +357,17 @@ int kt_gen_batch(uint32_t num_graphs, uint64_t seed, uint32_t n_min, uint32_t n_
 // METIS text: header "n m [fmt [ncon]]", then one 1-based adjacency line per
 // vertex; '%' lines are comments.  fmt digits: sizes / vertex weights / edge
 // weights (weights are skipped).
-int kt_read_metis(const char* path, kt_host_graph* out) {
-    if (!path || !out) return KT_GIO_EINVAL;
-    memset(out, 0, sizeof(*out));
-    FILE* f = fopen(path, "rb");
+static int read_metis_impl(const char* path, kt_host_graph* out, FILE*& f) {
+    f = fopen(path, "rb");
     if (!f) return KT_GIO_EIO;
+    // every head needs at least two bytes of the file ("1 "): the header's
+    // edge count only bounds the reservation, it is not trusted
+    uint64_t fsize = 0;
+    if (fseek(f, 0, SEEK_END) == 0) {
+        const long e = ftell(f);
+        fsize = e > 0 ? (uint64_t)e : 0;
+    }
+    rewind(f);
     std::string line;
     auto next_line = [&](std::string& s) -> bool {
         s.clear();
@@ -374,19 +381,13 @@ int kt_read_metis(const char* path, kt_host_graph* out) {
         return any;
     };
     do {
-        if (!next_line(line)) {
-            fclose(f);
-            return KT_GIO_EFORMAT;
-        }
+        if (!next_line(line)) return KT_GIO_EFORMAT;
     } while (!line.empty() && line[0] == '%');
     unsigned long long n = 0, m = 0;
     char fmt[16] = "0";
     unsigned ncon = 1;
     int got = sscanf(line.c_str(), "%llu %llu %15s %u", &n, &m, fmt, &ncon);
-    if (got < 2 || n > 0xffffffffull) {
-        fclose(f);
-        return KT_GIO_EFORMAT;
-    }
+    if (got < 2 || n > 0xffffffffull) return KT_GIO_EFORMAT;
     size_t fl = strlen(fmt);
     bool ew = fl >= 1 && fmt[fl - 1] == '1';
     bool vw = fl >= 2 && fmt[fl - 2] == '1';
@@ -394,7 +395,7 @@ int kt_read_metis(const char* path, kt_host_graph* out) {
     if (!vw) ncon = 0;
     std::vector<uint64_t> off(n + 1, 0);
     std::vector<uint32_t> heads;
-    heads.reserve(2 * m);
+    heads.reserve(std::min<uint64_t>(2 * (uint64_t)m, fsize / 2 + 1));
     uint64_t v = 0;
     int rc = KT_GIO_OK;
     while (v < n && next_line(line)) {
@@ -431,6 +432,7 @@ int kt_read_metis(const char* path, kt_host_graph* out) {
         ++v;
     }
     fclose(f);
+    f = nullptr;
     if (rc) return rc;
     for (; v < n; ++v) off[v + 1] = heads.size();
     out->n = (uint32_t)n;
@@ -446,6 +448,25 @@ int kt_read_metis(const char* path, kt_host_graph* out) {
     return KT_GIO_OK;
 }
 
+// No C++ exception leaves the C ABI: allocation failures (a header that
+// promises more vertices than memory holds) become KT_GIO_ENOMEM.
+int kt_read_metis(const char* path, kt_host_graph* out) {
+    if (!path || !out) return KT_GIO_EINVAL;
+    memset(out, 0, sizeof(*out));
+    FILE* f = nullptr;
+    int rc;
+    try {
+        rc = read_metis_impl(path, out, f);
+    } catch (const std::bad_alloc&) {
+        rc = KT_GIO_ENOMEM;
+    } catch (...) {
+        rc = KT_GIO_EFORMAT;
+    }
+    if (f) fclose(f);
+    if (rc) kt_host_graph_free(out);
+    return rc;
+}
+
 int kt_write_metis(const char* path, uint32_t n, const uint64_t* offsets, const uint32_t* heads) {
     if (!path || !offsets || (n && !heads)) return KT_GIO_EINVAL;
     FILE* f = fopen(path, "wb");
@@ -459,9 +480,7 @@ int kt_write_metis(const char* path, uint32_t n, const uint64_t* offsets, const 
     return fclose(f) == 0 ? KT_GIO_OK : KT_GIO_EIO;
 }
 
-int kt_read_graphio_binary(const char* first_out_path, const char* head_path, kt_host_graph* out) {
-    if (!first_out_path || !head_path || !out) return KT_GIO_EINVAL;
-    memset(out, 0, sizeof(*out));
+static int read_graphio_binary_impl(const char* first_out_path, const char* head_path, kt_host_graph* out) {
     auto slurp = [](const char* p, std::vector<uint64_t>& v) -> int {
         FILE* f = fopen(p, "rb");
         if (!f) return KT_GIO_EIO;
@@ -499,6 +518,21 @@ int kt_read_graphio_binary(const char* first_out_path, const char* head_path, kt
     memcpy(out->offsets, fo.data(), fo.size() * sizeof(uint64_t));
     for (size_t i = 0; i < hd.size(); ++i) out->heads[i] = (uint32_t)hd[i];
     return KT_GIO_OK;
+}
+
+int kt_read_graphio_binary(const char* first_out_path, const char* head_path, kt_host_graph* out) {
+    if (!first_out_path || !head_path || !out) return KT_GIO_EINVAL;
+    memset(out, 0, sizeof(*out));
+    int rc;
+    try {
+        rc = read_graphio_binary_impl(first_out_path, head_path, out);
+    } catch (const std::bad_alloc&) {
+        rc = KT_GIO_ENOMEM;
+    } catch (...) {
+        rc = KT_GIO_EFORMAT;
+    }
+    if (rc) kt_host_graph_free(out);
+    return rc;
 }
 
 int kt_write_graphio_binary(const char* first_out_path, const char* head_path, uint32_t n, const uint64_t* offsets,

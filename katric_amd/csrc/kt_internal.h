@@ -12,6 +12,8 @@
 #define KT_FLAG_UNSORTED 0x100u  // some list not ascending (not an error)
 #define KT_ERR_GHOST 0x10u       // shard: remote head missing from the ghost table
 #define KT_ERR_MESSAGE 0x20u     // shard: malformed message list
+#define KT_ERR_SEGMENT 0x40u     // batch: a graph's per-vertex sum is not a multiple of 3
+#define KT_ERR_ASYMMETRIC 0x80u  // some slot v -> u has no slot u -> v (pair checksum or kept count)
 #define KT_ERR_MASK 0xffu
 
 namespace kt {
@@ -23,8 +25,27 @@ uint64_t radix_tmp_words(uint64_t len);
 void radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t len, int bits, uint64_t* tmp, hipStream_t st);
 void tile_bounds_launch(uint32_t n, uint64_t nnz, uint64_t tile, const uint64_t* off, uint32_t* bnd, hipStream_t st);
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz);
+// Validation of a symmetric CSR beside the orientation count pass.  Every
+// slot v -> u adds +h(min, max) (v < u) or -h(min, max) (v > u) of the
+// unordered pair to two 64-bit sums, h being two independently keyed 64-bit
+// mixes; without duplicates in a row, a symmetric CSR sums to zero and an
+// asymmetric one does not, except when both keyed sums collide (probability
+// about 2^-128 for a key drawn at random).  With vbase (G + 1 entries, a
+// disjoint union of G graphs), a head outside its row's graph is an error.
+struct OrientCheck {
+    uint64_t key[2];               // hash keys
+    const uint64_t* vbase = nullptr;
+    uint32_t G = 0;
+};
 void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,
-                         uint64_t* ooff, uint64_t* ws, uint32_t* err, hipStream_t st);
+                         uint64_t* ooff, uint64_t* ws, uint32_t* err, const OrientCheck& chk, hipStream_t st);
+// Pair checksum, self-loop, range and (sorted-row) duplicate checks of a whole
+// symmetric CSR without orienting it; ws holds sym_ws_words(n, nnz) words.
+uint64_t sym_ws_words(uint32_t n, uint64_t nnz);
+void sym_check_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint64_t* ws,
+                      uint32_t* err, const OrientCheck& chk, hipStream_t st);
+// A random key pair for OrientCheck.
+void random_keys(uint64_t key[2]);
 void orient_write_launch(uint32_t n, uint64_t nnz, const uint32_t* heads, uint32_t* oheads, uint64_t* ws,
                          hipStream_t st);
 void make_keys(uint32_t n, uint64_t m, const uint64_t* ooff, const uint32_t* oheads, uint64_t* keys,
@@ -46,6 +67,8 @@ struct TcParams {
     uint32_t order;           // TC_BY_DEGREE or TC_BY_VERTEX (work-queue order)
     uint32_t table_shift = 4; // hash tables hold >= 2^table_shift slots per key where they fit
     uint32_t wave_grab = 32;  // wave-bin tails a wave takes per queue access (1..64), vertex order
+    uint32_t bsearch_min_deg = 8192;  // |S(x)| above this: binary-search probes in global memory (<= 8192)
+    uint32_t pseudo2core = 0; // skip tails with |S(x)| < 2 and count them (Config::pseudo2core)
 };
 
 // How a team walks the T lists of a chunk of neighbours: the whole wave on
@@ -127,7 +150,8 @@ struct BatchBufs {
 uint64_t batch_layout(void* base, uint32_t G, uint64_t N, uint64_t nnz, bool host_in, bool own_pv, bool unsorted,
                       BatchBufs& B);
 hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
-                    uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort);
+                          uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort,
+                          const uint64_t sym_key[2]);
 
 uint64_t tc_ws_words(uint32_t X);
 // Enqueues binning and the intersection kernels; returns the launch status.

@@ -11,6 +11,7 @@
 // with (deg v, v) < (deg u, u), in input order (ascending ids when the input
 // lists are sorted; otherwise they are sorted afterwards by radix sort).
 #include <algorithm>
+#include <random>
 
 #include "kt_device.h"
 #include "kt_internal.h"
@@ -163,11 +164,53 @@ __global__ void k_tile_bounds(uint32_t n, uint64_t nnz, uint64_t tile, const uin
     }
 }
 
+// Graph bounds of a tile's rows for the batch range check: the graphs of its
+// first and last row, and the vertex range of the first.
+struct TileGraphs {
+    uint32_t glo, ghi;
+    uint64_t a, b;
+};
+
+__device__ __forceinline__ TileGraphs tile_graphs(const uint64_t* __restrict__ vbase, uint32_t G, uint32_t lo,
+                                                  uint32_t hi) {
+    TileGraphs t;
+    t.glo = graph_of(vbase, 0, G - 1, lo);
+    t.ghi = graph_of(vbase, t.glo, G - 1, hi);
+    t.a = vbase[t.glo];
+    t.b = vbase[t.glo + 1];
+    return t;
+}
+
+// A head u of row v lies outside v's graph.
+__device__ __forceinline__ bool outside_graph(const uint64_t* __restrict__ vbase, const TileGraphs& t, uint32_t v,
+                                              uint32_t u) {
+    if (t.glo == t.ghi) return u < t.a || u >= t.b;
+    const uint32_t g = graph_of(vbase, t.glo, t.ghi, v);
+    return u < vbase[g] || u >= vbase[g + 1];
+}
+
+// Adds a thread's two checksum terms to sym[0..1] (one atomic per wave).
+__device__ __forceinline__ void sym_flush(uint64_t s0, uint64_t s1, uint64_t* __restrict__ sym) {
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if (lane_id() == 0) {
+        if (s0) atomic_add_u64(&sym[0], s0);
+        if (s1) atomic_add_u64(&sym[1], s1);
+    }
+}
+
+struct CheckArgs {
+    uint64_t k0, k1;
+    const uint64_t* vbase;
+    uint32_t G;
+};
+
 __global__ void __launch_bounds__(OR_THREADS)
     k_orient_keep(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
                   const uint32_t* __restrict__ deg, const uint8_t* __restrict__ dcls, const uint32_t* __restrict__ tbnd,
                   uint64_t* __restrict__ keep_bits,
-                  uint64_t* __restrict__ dplus, uint64_t* __restrict__ tile_cnt, uint32_t* __restrict__ err) {
+                  uint64_t* __restrict__ dplus, uint64_t* __restrict__ tile_cnt, uint32_t* __restrict__ err,
+                  CheckArgs ck, uint64_t* __restrict__ sym) {
     __shared__ uint32_t marks[OR_TILE];
     __shared__ uint32_t rowcnt[OR_TILE];
     __shared__ uint32_t wmax[OR_THREADS / KT_WAVE];
@@ -181,7 +224,10 @@ __global__ void __launch_bounds__(OR_THREADS)
     tile_rows<OR_THREADS, OR_ITEMS>(off, n, tile0, tend, marks, bnd, wmax, lo, hi, tbnd);
     const uint64_t nrows = (uint64_t)hi - lo + 1;
     const bool lds_rows = nrows <= OR_TILE;
+    TileGraphs tg{};
+    if (ck.vbase) tg = tile_graphs(ck.vbase, ck.G, lo, hi);
     uint32_t flags = 0, cnt = 0;
+    uint64_t s0 = 0, s1 = 0;
     // The 64 slots of a wave are consecutive: the next slot's head and row
     // come from the next lane (the last lane loads them), and kept slots are
     // counted per run of equal rows with one LDS atomic from the run's last lane.
@@ -212,13 +258,15 @@ __global__ void __launch_bounds__(OR_THREADS)
         const uint32_t v = lo + rr;
         const uint32_t u = hu[k];
         if (in) {
-            if (u >= n) {
+            if (u >= n || (ck.vbase && outside_graph(ck.vbase, tg, v, u))) {
                 flags |= KT_ERR_HEAD_RANGE;
             } else {
                 if (u == v) flags |= KT_ERR_SELF_LOOP;
                 // classes decide unless they are equal above 127 (then the
                 // exact degrees, a rare gather)
                 keep = class_before(cv[k], v, cu[k], u, deg);
+                s0 += pair_term(v, u, ck.k0);
+                s1 += pair_term(v, u, ck.k1);
             }
         }
         // next slot in the same row? (shuffles outside any branch: every lane active)
@@ -257,6 +305,7 @@ __global__ void __launch_bounds__(OR_THREADS)
             keep_bits[(tile0 + k * OR_THREADS + (tid & ~63u)) / 64] = bits;
     }
     if (flags) atomicOr(err, flags);
+    sym_flush(s0, s1, sym);
     __syncthreads();
     if (lds_rows) {
         // rows wholly inside the tile get a plain store, the two boundary rows an atomic
@@ -273,6 +322,68 @@ __global__ void __launch_bounds__(OR_THREADS)
     uint64_t tot;
     block_excl_sum(cnt, lw, &tot);
     if (tid == 0) tile_cnt[blockIdx.x] = tot;
+}
+
+// Folds the pair checksums and the kept count into the error flags: a
+// symmetric CSR keeps exactly half of its slots and its sums are zero.
+__global__ void k_sym_final(const uint64_t* __restrict__ sym, const uint64_t* __restrict__ kept, uint64_t nnz,
+                            uint32_t* __restrict__ err) {
+    if (threadIdx.x != 0) return;
+    bool bad = (sym[0] | sym[1]) != 0;
+    if (kept) bad |= 2 * kept[0] != nnz;
+    if (bad) atomicOr(err, KT_ERR_ASYMMETRIC);
+}
+
+// Validation of a whole CSR without orienting it (rows resolved per tile as
+// in k_orient_keep): head range, self-loops, adjacent duplicates of sorted
+// rows, and the pair checksums.
+__global__ void __launch_bounds__(OR_THREADS)
+    k_sym_tiles(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
+                const uint32_t* __restrict__ tbnd, uint32_t* __restrict__ err, CheckArgs ck,
+                uint64_t* __restrict__ sym) {
+    __shared__ uint32_t marks[OR_TILE];
+    __shared__ uint32_t wmax[OR_THREADS / KT_WAVE];
+    __shared__ uint32_t bnd[2];
+    const uint64_t tile0 = (uint64_t)blockIdx.x * OR_TILE;
+    const uint64_t tend = min(tile0 + OR_TILE, nnz);
+    const unsigned tid = threadIdx.x, lane = lane_id();
+    uint32_t lo, hi;
+    tile_rows<OR_THREADS, OR_ITEMS>(off, n, tile0, tend, marks, bnd, wmax, lo, hi, tbnd);
+    uint32_t flags = 0;
+    uint64_t s0 = 0, s1 = 0;
+#pragma unroll 4
+    for (int k = 0; k < OR_ITEMS; ++k) {
+        const unsigned i = k * OR_THREADS + tid;
+        const uint64_t sl = tile0 + i;
+        const bool in = sl < tend;
+        const uint32_t rr = in ? marks[i] : 0xffffffffu;
+        const uint32_t v = lo + rr;
+        const uint32_t u = in ? heads[sl] : 0u;
+        if (in) {
+            if (u >= n) {
+                flags |= KT_ERR_HEAD_RANGE;
+            } else {
+                if (u == v) flags |= KT_ERR_SELF_LOOP;
+                s0 += pair_term(v, u, ck.k0);
+                s1 += pair_term(v, u, ck.k1);
+            }
+        }
+        uint32_t nx = __shfl_down(u, 1, 64);
+        bool same = __shfl_down(rr, 1, 64) == rr;
+        if (lane == KT_WAVE - 1) {
+            same = in && ((i + 1 < OR_TILE) ? (sl + 1 < tend && marks[i + 1] == rr)
+                                            : (sl + 1 < nnz && sl + 1 < off[v + 1]));
+            if (same) nx = heads[sl + 1];
+        }
+        if (in && same) {
+            if (nx == u)
+                flags |= KT_ERR_DUPLICATE;
+            else if (nx < u)
+                flags |= KT_FLAG_UNSORTED;
+        }
+    }
+    if (flags) atomicOr(err, flags);
+    sym_flush(s0, s1, sym);
 }
 
 // Copies the kept heads in slot order: output position = tile base + rank of
@@ -452,6 +563,7 @@ void radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t len, int bits, uint6
 // ---------------------------------------------------------------------------
 struct OrientWs {
     uint64_t tiles;
+    uint64_t* sym;       // 2: pair checksums
     uint64_t* tile_cnt;  // tiles + 1
     uint64_t* tile_off;  // tiles + 1 (tile_off[tiles] = kept total)
     uint64_t* bits;      // ceil(nnz / 64) + 1
@@ -464,7 +576,8 @@ struct OrientWs {
 static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
     OrientWs w;
     w.tiles = (nnz + OR_TILE - 1) / OR_TILE;
-    w.tile_cnt = ws;
+    w.sym = ws;
+    w.tile_cnt = ws + 2;
     w.tile_off = w.tile_cnt + w.tiles + 1;
     w.bits = w.tile_off + w.tiles + 1;
     w.dplus = w.bits + (nnz + 63) / 64 + 1;
@@ -477,7 +590,7 @@ static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
 
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz) {
     uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
-    return 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 + tiles +
+    return 2 + 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 + tiles +
            std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1)) + ((uint64_t)n + 7) / 8;
 }
 
@@ -493,20 +606,53 @@ void tile_bounds_launch(uint32_t n, uint64_t nnz, uint64_t tile, const uint64_t*
 // Degrees, validation flags, keep bitmask, d+ per row and its scan into ooff
 // (ooff[n] = kept total), per-tile kept counts and their scan.
 void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,
-                         uint64_t* ooff, uint64_t* ws, uint32_t* err, hipStream_t st) {
+                         uint64_t* ooff, uint64_t* ws, uint32_t* err, const OrientCheck& chk, hipStream_t st) {
     OrientWs w = orient_ws(n, nnz, ws);
     k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, w.dcls, err);
     fill_u64(w.dplus, (uint64_t)n + 1, 0, st);
+    fill_u64(w.sym, 2, 0, st);
     if (w.tiles > 0) {
+        const CheckArgs ck{chk.key[0], chk.key[1], chk.G ? chk.vbase : nullptr, chk.G};
         fill_u32(w.tbnd, 2 * w.tiles, 0, st);
         k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, w.tbnd);
         k_orient_keep<<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.dcls, w.tbnd, w.bits, w.dplus,
-                                                               w.tile_cnt, err);
+                                                               w.tile_cnt, err, ck, w.sym);
         scan_u64(w.tile_cnt, w.tile_off, w.tiles, w.stmp, st);
     } else {
         fill_u64(w.tile_off, 1, 0, st);
     }
     scan_u64(w.dplus, ooff, n, w.stmp, st);
+    k_sym_final<<<1, 64, 0, st>>>(w.sym, ooff + n, nnz, err);
+}
+
+uint64_t sym_ws_words(uint32_t n, uint64_t nnz) {
+    (void)n;
+    const uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
+    return 2 + tiles + 1;
+}
+
+void sym_check_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint64_t* ws,
+                      uint32_t* err, const OrientCheck& chk, hipStream_t st) {
+    const uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
+    uint64_t* sym = ws;
+    uint32_t* tbnd = reinterpret_cast<uint32_t*>(ws + 2);
+    fill_u64(sym, 2, 0, st);
+    if (tiles > 0 && n > 0) {
+        const CheckArgs ck{chk.key[0], chk.key[1], nullptr, 0};
+        fill_u32(tbnd, 2 * tiles, 0, st);
+        k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, tbnd);
+        k_sym_tiles<<<(unsigned)tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, tbnd, err, ck, sym);
+    }
+    k_sym_final<<<1, 64, 0, st>>>(sym, nullptr, nnz, err);
+}
+
+void random_keys(uint64_t key[2]) {
+    std::random_device rd;
+    for (int i = 0; i < 2; ++i) {
+        uint64_t k = 0;
+        for (int j = 0; j < 4; ++j) k = (k << 16) ^ (uint64_t)(rd() & 0xffffu);
+        key[i] = k ^ (0x9e3779b97f4a7c15ull * (uint64_t)(i + 1));
+    }
 }
 
 // Writes the oriented heads; only valid after orient_count_launch on the same

@@ -120,6 +120,7 @@ struct ServeState {
     int device = 0;
     int num_cus = 0;
     TcParams prm{};
+    uint64_t sym_key[2] = {0, 0};
     uint64_t slot_cap = 0;
     std::thread worker;
     std::mutex mu;
@@ -134,10 +135,15 @@ struct ServeState {
     char err[256] = {0};
 };
 
-static uint64_t req_bytes(const ServeReq* r) {
+// Device bytes of a slot's batch: the batch pipeline's buffers (with the
+// radix-sort scratch for a relaunch of unsorted lists) and the gather
+// descriptors after them.
+static uint64_t slot_bytes(uint32_t G, uint64_t N, uint64_t nnz, bool sort) {
     BatchBufs B;
-    return batch_layout(nullptr, 1, r->n, r->nnz, true, true, false, B) + 512;
+    return ((batch_layout(nullptr, G, N, nnz, true, true, sort, B) + 255) & ~255ull) + (uint64_t)G * sizeof(GatherDesc);
 }
+
+static uint64_t req_bytes(const ServeReq* r) { return slot_bytes(1, r->n, r->nnz, false); }
 
 static int slot_grow(ServeSlot& s, uint64_t bytes, uint32_t G) {
     if (bytes > s.arena_bytes) {
@@ -178,6 +184,9 @@ static int launch(ServeState* S, ServeSlot& sl, bool sort) {
     }
     BatchBufs B;
     const uint64_t used = (batch_layout(nullptr, G, N, nnz, true, true, sort, B) + 255) & ~255ull;
+    // a slot never holds more than its share of the budget (a relaunch with
+    // the sort scratch can need more than admission counted)
+    if (slot_bytes(G, N, nnz, sort) > S->slot_cap) return KT_ENOMEM;
     int rc = slot_grow(sl, used + (uint64_t)G * sizeof(GatherDesc), G);
     if (rc) return rc;
     batch_layout(sl.arena, G, N, nnz, true, true, sort, B);
@@ -217,7 +226,8 @@ static int launch(ServeState* S, ServeSlot& sl, bool sort) {
         return KT_EHIP;
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, (uint64_t)S->num_cus * 4));
     k_serve_gather<<<grid, 256, 0, st>>>(G, d_desc, chunks, d_off, d_heads, N, nnz);
-    if (batch_pipeline(st, S->num_cus, S->prm, B, G, N, nnz, B.pv, sl.h_totals, sl.h_small, sort) != hipSuccess)
+    if (batch_pipeline(st, S->num_cus, S->prm, B, G, N, nnz, B.pv, sl.h_totals, sl.h_small, sort, S->sym_key) !=
+        hipSuccess)
         return KT_EHIP;
     for (uint32_t g = 0; g < G; ++g) {
         ServeReq* r = sl.reqs[g];
@@ -268,6 +278,29 @@ static void finish(ServeState* S, ServeSlot& sl) {
     S->cv_done.notify_all();
 }
 
+// A batch that could not be launched (lock held by the caller): a batch of
+// several requests that ran out of device memory is re-run one request at a
+// time, so that only a request too large on its own fails; any other error
+// fails the batch's requests.
+static void launch_failed(ServeState* S, ServeSlot& sl, int rc) {
+    const uint32_t G = (uint32_t)sl.reqs.size();
+    if (rc == KT_ENOMEM && G > 1) {
+        for (int g = (int)G - 1; g >= 0; --g) {
+            sl.reqs[g]->isolate = true;
+            S->pending.push_front(sl.reqs[g]);
+        }
+        S->st.isolation_reruns++;
+    } else {
+        for (auto* r : sl.reqs) {
+            r->status = rc;
+            r->done = true;
+        }
+    }
+    sl.reqs.clear();
+    sl.busy = false;
+    S->cv_done.notify_all();
+}
+
 static void worker_main(ServeState* S) {
     (void)hipSetDevice(S->device);
     std::unique_lock<std::mutex> lk(S->mu);
@@ -282,14 +315,7 @@ static void worker_main(ServeState* S) {
                 lk.unlock();
                 int rc = launch(S, sl, true);  // same requests, with the radix sort
                 lk.lock();
-                if (rc == KT_OK) continue;
-                for (auto* r : sl.reqs) {
-                    r->status = rc;
-                    r->done = true;
-                }
-                sl.reqs.clear();
-                sl.busy = false;
-                S->cv_done.notify_all();
+                if (rc != KT_OK) launch_failed(S, sl, rc);
                 continue;
             }
             finish(S, sl);
@@ -321,10 +347,7 @@ static void worker_main(ServeState* S) {
                     continue;
                 }
                 if (!batch.empty() && (r->isolate || batch.front()->isolate)) break;
-                BatchBufs B;
-                const uint64_t bytes = batch_layout(nullptr, (uint32_t)batch.size() + 1, N + r->n, nnz + r->nnz, true,
-                                                    true, false, B) +
-                                       (batch.size() + 2) * 8 + 256;
+                const uint64_t bytes = slot_bytes((uint32_t)batch.size() + 1, N + r->n, nnz + r->nnz, false);
                 const bool limits = batch.size() + 1 <= S->cfg.max_batch_graphs &&
                                     (nnz + r->nnz) / 2 <= S->cfg.max_batch_edges;
                 if (!batch.empty() && (bytes > S->slot_cap || !limits)) break;
@@ -345,15 +368,7 @@ static void worker_main(ServeState* S) {
             lk.unlock();
             int rc = launch(S, *free_slot, false);
             lk.lock();
-            if (rc != KT_OK) {
-                for (auto* r : free_slot->reqs) {
-                    r->status = rc;
-                    r->done = true;
-                }
-                free_slot->reqs.clear();
-                free_slot->busy = false;
-                S->cv_done.notify_all();
-            }
+            if (rc != KT_OK) launch_failed(S, *free_slot, rc);
             continue;
         }
         bool any_busy = S->slots[0].busy || S->slots[1].busy;
@@ -430,6 +445,8 @@ extern "C" int kt_serve_start(kt_ctx* c, const kt_serve_config* cfg) {
     S->device = c->device;
     S->num_cus = c->num_cus;
     S->prm = c->prm;
+    S->sym_key[0] = c->sym_key[0];
+    S->sym_key[1] = c->sym_key[1];
     for (auto& sl : S->slots) {
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||

@@ -435,6 +435,10 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
     if (!in || !out || !in->range_bounds || !in->offsets || in->num_ranks == 0 || in->rank >= in->num_ranks ||
         in->algorithm > KT_SHARD_DITRIC)
         return c->fail(KT_EINVAL, "bad shard arguments");
+    // with pseudo2core the reference's senders also drop skipped interface
+    // nodes from the distributed phase (cetric_edge_iterator.h:1247-1271),
+    // and the messages are this call's inputs: not supported on shards
+    if (c->prm.pseudo2core) return c->fail(KT_EINVAL, "pseudo2core applies to the shared-memory path only");
     const uint32_t P = in->num_ranks;
     const uint64_t* bd = in->range_bounds;
     if (bd[0] != 0) return c->fail(KT_EINVAL, "range_bounds[0] != 0");

@@ -346,6 +346,7 @@ struct ProdBufs {
     uint64_t* gpv;
     uint64_t* hoff;
     uint32_t* hheads;
+    uint64_t* symws;  // symmetry check of the whole graph
 };
 
 uint64_t prod_layout(void* base, uint32_t n, uint64_t nnz, uint32_t P, uint32_t nl, uint64_t ng_max, bool want_pv,
@@ -374,6 +375,7 @@ uint64_t prod_layout(void* base, uint32_t n, uint64_t nnz, uint32_t P, uint32_t 
     B.gpv = host_pv ? static_cast<uint64_t*>(cv.take((uint64_t)n * 8)) : nullptr;
     B.hoff = host_in ? static_cast<uint64_t*>(cv.take(((uint64_t)n + 1) * 8)) : nullptr;
     B.hheads = host_in ? static_cast<uint32_t*>(cv.take(std::max<uint64_t>(nnz, 1) * 4)) : nullptr;
+    B.symws = static_cast<uint64_t*>(cv.take(sym_ws_words(n, nnz) * 8));
     return cv.used;
 }
 
@@ -388,6 +390,7 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
     if (!out || !offsets || !bd || P == 0 || rank >= P || algorithm > KT_SHARD_DITRIC)
         return c->fail(KT_EINVAL, "bad shard arguments");
     if (bd[0] != 0 || bd[P] != n) return c->fail(KT_EINVAL, "range_bounds must cover [0, n)");
+    if (c->prm.pseudo2core) return c->fail(KT_EINVAL, "pseudo2core applies to the shared-memory path only");
     for (uint32_t r = 0; r < P; ++r)
         if (bd[r + 1] < bd[r]) return c->fail(KT_EINVAL, "range bounds not ascending");
     out->local_triangles = out->type3_triangles = 0;
@@ -445,6 +448,10 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
     fill_u32(B.bits, 2 * W, 0, st);
     fill_u64(B.voff, ng_max + 1, 0, st);
     k_pd_degree<<<grid_for(n), 256, 0, st>>>(n, nnz, off, B.deg, err);
+    // the whole graph must be symmetric, as every rank's view of it assumes
+    // (one pass over all slots: pair checksums, self-loops, duplicates)
+    const OrientCheck chk{{c->sym_key[0], c->sym_key[1]}};
+    sym_check_launch(n, nnz, off, hd, B.symws, err, chk, st);
     // The kernels up to the first flag check stay in bounds whatever the
     // offsets hold: slots are bounded by the host-checked slice [base, end),
     // rows by tile_rows' clamping, ids by n.
@@ -471,8 +478,10 @@ int count_shard_graph_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, con
     uint32_t fl = (uint32_t)c->host_small[0];
     const uint64_t ng = c->host_small[1], nmsg = c->host_small[2], V = c->host_small[3];
     if (fl & KT_ERR_MASK)
-        return c->fail(KT_EINVAL, "malformed CSR:%s%s", (fl & KT_ERR_OFFSETS) ? " bad offsets" : "",
-                       (fl & KT_ERR_HEAD_RANGE) ? " head id >= n" : "");
+        return c->fail(KT_EINVAL, "malformed CSR:%s%s%s%s%s", (fl & KT_ERR_OFFSETS) ? " bad offsets" : "",
+                       (fl & KT_ERR_HEAD_RANGE) ? " head id >= n" : "", (fl & KT_ERR_SELF_LOOP) ? " self-loop" : "",
+                       (fl & KT_ERR_DUPLICATE) ? " duplicate edge" : "",
+                       (fl & KT_ERR_ASYMMETRIC) ? " not symmetric" : "");
     if (ng > ng_max || nmsg > ng || V > nnz) return c->fail(KT_EINVAL, "shard producer: inconsistent ghost counts");
     // message lists: count, scan, emit over the virtual slot space
     uint64_t mtot = 0;

@@ -63,7 +63,7 @@ constexpr uint32_t WAVE_GRAB = 8;  // wave-phase tails taken per queue access
 //   Q_FILL + 32 * b       fill cursor of bucket b (binning pass 2)
 enum : int { Q_CNT = 0, Q_PULL = 32, Q_FILL = 32 * (NBUCKET + 1), Q_STRIDE = 32, Q_WORDS = 32 * (2 * NBUCKET + 2) };
 // stats words after the queue header (the reference's work counters)
-enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_WORDS = 4 };
+enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_SKIP = 2, ST_WORDS = 4 };
 
 template <bool PLAIN>
 struct View {
@@ -136,8 +136,9 @@ __device__ __forceinline__ int tail_bucket(const View<PLAIN>& V, const TcParams&
     V.r_range(x, rb, re);
     s = se - sb;
     r = re - rb;
-    if (s == 0 || r == 0 || (PLAIN && s < 2)) return -1;
-    if (s > prm.group_max_deg) return s > BIG_SLOTS / 2 ? BUCKET_HUGE : BUCKET_BIG;
+    if (s == 0 || r == 0 || ((PLAIN || prm.pseudo2core) && s < 2)) return -1;
+    if (s > prm.bsearch_min_deg) return BUCKET_HUGE;
+    if (s > prm.group_max_deg) return BUCKET_BIG;
     const int b = (int)ceil_log2(s);  // 0..11
     if (prm.order == TC_BY_VERTEX) {  // two classes per XCD, each in vertex order
         const int xcd = (int)(((uint64_t)x * NXCD) / V.v.X);
@@ -152,17 +153,20 @@ template <bool PLAIN>
 __global__ void __launch_bounds__(256) k_tc_bin_count(View<PLAIN> V, TcParams prm, uint32_t* __restrict__ q,
                                                       uint64_t* __restrict__ stats) {
     __shared__ uint32_t cnt[NBUCKET];
-    __shared__ uint64_t red[2][4];
+    __shared__ uint64_t red[3][4];
     if (threadIdx.x < NBUCKET) cnt[threadIdx.x] = 0;
     __syncthreads();
-    uint64_t wedge = 0, isz = 0;
+    uint64_t wedge = 0, isz = 0, skipped = 0;
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < V.v.X; x += gridDim.x * blockDim.x) {
         uint64_t s, r;
         int b = tail_bucket(V, prm, x, s, r);
         if (b >= 0) atomicAdd(&cnt[b], 1u);
+        // pseudo2core: tails with fewer than two out-neighbours are skipped
+        // and counted (cetric_edge_iterator.h:239-242)
+        if (prm.pseudo2core && s < 2) ++skipped;
         // a plain tail with |N+(x)| == 1 emits nothing but still counts one
         // intersection in the reference's counters (cetric_edge_iterator.h:279-281)
-        if (PLAIN && s == 1) {
+        if (PLAIN && s == 1 && !prm.pseudo2core) {
             uint64_t sb, se, tb, te;
             uint32_t u;
             V.s_range(x, sb, se);
@@ -188,16 +192,20 @@ __global__ void __launch_bounds__(256) k_tc_bin_count(View<PLAIN> V, TcParams pr
     }
     wedge = wave_sum(wedge);
     isz = wave_sum(isz);
+    skipped = wave_sum(skipped);
     if (lane_id() == 0) {
         red[0][threadIdx.x / KT_WAVE] = wedge;
         red[1][threadIdx.x / KT_WAVE] = isz;
+        red[2][threadIdx.x / KT_WAVE] = skipped;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
         uint64_t c = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        uint64_t d = red[2][0] + red[2][1] + red[2][2] + red[2][3];
         if (a) atomic_add_u64(&stats[ST_WEDGE], a);
         if (c) atomic_add_u64(&stats[ST_ISZ], c);
+        if (d) atomic_add_u64(&stats[ST_SKIP], d);
     }
     if (threadIdx.x < NBUCKET && cnt[threadIdx.x]) atomicAdd(&q[Q_CNT + threadIdx.x], cnt[threadIdx.x]);
 }
@@ -927,6 +935,7 @@ static hipError_t launch_impl(const TcView& view, const TcOut& out, const TcPara
     TcParams p = prm;
     if (p.wave_max_deg > WAVE_SLOTS / 2) p.wave_max_deg = WAVE_SLOTS / 2;
     if (p.group_max_deg > GROUP_SLOTS / 2) p.group_max_deg = GROUP_SLOTS / 2;
+    if (p.bsearch_min_deg > BIG_SLOTS / 2) p.bsearch_min_deg = BIG_SLOTS / 2;
     // bucket edges are powers of two: round the wave threshold up to one
     uint32_t wm = 1;
     while (wm < p.wave_max_deg) wm <<= 1;

@@ -70,7 +70,7 @@ def local_statistics(stats: dict, timing: dict, rank: int = 0, wedge_count: int 
         "wedges": int(wedge_count),
         "local_wall_time": total,
         "local_time": total,
-        "skipped_nodes": 0,
+        "skipped_nodes": int(stats.get("skipped_nodes", 0)),
         "wedge_checks": int(stats.get("wedge_checks", 0)),
         "intersection_size_local": int(stats.get("intersection_size_local", 0)),
         "intersection_size_global": int(stats.get("intersection_size_global", 0)),

@@ -264,16 +264,21 @@ int or_orient(uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint64
 
 int or_run_shmem(uint32_t n, const uint64_t* offsets, const uint32_t* heads, int method, int threads,
                  uint64_t* total, uint64_t* per_vertex, or_stats* stats) {
+    return or_run_shmem_ex(n, offsets, heads, method, threads, 0, total, per_vertex, stats);
+}
+
+int or_run_shmem_ex(uint32_t n, const uint64_t* offsets, const uint32_t* heads, int method, int threads,
+                    int pseudo2core, uint64_t* total, uint64_t* per_vertex, or_stats* stats) {
     uint64_t* fo = (uint64_t*)malloc(((size_t)n + 1) * sizeof(uint64_t));
     uint32_t* h = orient_full(n, offsets, heads, fo, threads);
     if (per_vertex) memset(per_vertex, 0, (size_t)n * sizeof(uint64_t));
-    uint64_t tri = 0, wedge = 0, isz = 0;
+    uint64_t tri = 0, wedge = 0, isz = 0, skipped = 0;
     const int atomic = threads > 1;
     /* run_local_sequential (cetric_edge_iterator.h:229-298) / node_loop_body
      * of run_local_parallel (:993-1023): for every v, for every u in N+(v),
      * intersect N+(v) with N+(u) and emit {v,u,w} per common w.  offset = 0:
      * NodeOrdering (id_outward) differs from EdgeOrientationOrdering. */
-#pragma omp parallel num_threads(threads > 1 ? threads : 1) reduction(+ : tri, wedge, isz)
+#pragma omp parallel num_threads(threads > 1 ? threads : 1) reduction(+ : tri, wedge, isz, skipped)
     {
         uint32_t* buf = NULL;
         size_t cap = 0;
@@ -282,6 +287,11 @@ int or_run_shmem(uint32_t n, const uint64_t* offsets, const uint32_t* heads, int
             uint32_t v = (uint32_t)vv;
             const uint32_t* nv = h + offsets[v] + fo[v];
             uint64_t dv = offsets[v + 1] - offsets[v] - fo[v];
+            /* pseudo2core: cetric_edge_iterator.h:239-242 */
+            if (pseudo2core && dv < 2) {
+                skipped++;
+                continue;
+            }
             for (uint64_t i = 0; i < dv; ++i) {
                 uint32_t u = nv[i];
                 const uint32_t* nu = h + offsets[u] + fo[u];
@@ -315,9 +325,51 @@ int or_run_shmem(uint32_t n, const uint64_t* offsets, const uint32_t* heads, int
         stats->wedge_checks = wedge;
         stats->intersection_size_local = isz;
         stats->local_triangles = tri;
+        stats->skipped_nodes = skipped;
     }
     free(h);
     free(fo);
+    return 0;
+}
+
+/* One-thread shmetric over a sample of tails, for timing: preprocessing
+ * (orient + sort) of the whole graph, then run_local_sequential
+ * (cetric_edge_iterator.h:229-298) over the tails v with v % stride == 0.
+ * Reports both phases' seconds and the oriented edges of the sampled tails. */
+int or_run_shmem_sampled(uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t stride,
+                         uint64_t* sample_triangles, uint64_t* sample_edges, double* prep_s, double* local_s) {
+    if (stride == 0) return -1;
+    double t0 = omp_get_wtime();
+    uint64_t* fo = (uint64_t*)malloc(((size_t)n + 1) * sizeof(uint64_t));
+    uint32_t* h = orient_full(n, offsets, heads, fo, 1);
+    double t1 = omp_get_wtime();
+    uint64_t tri = 0, edges = 0;
+    uint32_t* buf = NULL;
+    size_t cap = 0;
+    for (uint64_t v = 0; v < n; v += stride) {
+        const uint32_t* nv = h + offsets[v] + fo[v];
+        uint64_t dv = offsets[v + 1] - offsets[v] - fo[v];
+        edges += dv;
+        for (uint64_t i = 0; i < dv; ++i) {
+            uint32_t u = nv[i];
+            const uint32_t* nu = h + offsets[u] + fo[u];
+            uint64_t du = offsets[u + 1] - offsets[u] - fo[u];
+            size_t need = dv < du ? dv : du;
+            if (need > cap) {
+                cap = need * 2;
+                buf = (uint32_t*)realloc(buf, cap * sizeof(uint32_t));
+            }
+            tri += isect_u32(nv, dv, nu, du, buf, OR_MERGE, 1000, 1.0);
+        }
+    }
+    double t2 = omp_get_wtime();
+    free(buf);
+    free(h);
+    free(fo);
+    *sample_triangles = tri;
+    *sample_edges = edges;
+    *prep_s = t1 - t0;
+    *local_s = t2 - t1;
     return 0;
 }
 

@@ -68,6 +68,17 @@ int or_orient(uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint64
  * emitted Triangle{v,u,w}.  stats nullable. */
 int or_run_shmem(uint32_t n, const uint64_t* offsets, const uint32_t* heads, int method, int threads,
                  uint64_t* total, uint64_t* per_vertex, or_stats* stats);
+/* The same with Config::pseudo2core (config.h:184): tails with |N+(v)| < 2
+ * are skipped and counted in stats->skipped_nodes
+ * (cetric_edge_iterator.h:239-242). */
+int or_run_shmem_ex(uint32_t n, const uint64_t* offsets, const uint32_t* heads, int method, int threads,
+                    int pseudo2core, uint64_t* total, uint64_t* per_vertex, or_stats* stats);
+
+/* Timing helper: one-thread preprocessing of the whole graph, then the local
+ * phase over the tails v % stride == 0 only (a sample of the workload);
+ * returns the sample's triangles and oriented edges and both phases' seconds. */
+int or_run_shmem_sampled(uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t stride,
+                         uint64_t* sample_triangles, uint64_t* sample_edges, double* prep_s, double* local_s);
 
 /* Brute force over adjacency bitsets (n <= 4096): every triple a<b<c with all
  * three edges present.  Independent of the orientation machinery. */

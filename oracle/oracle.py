@@ -45,7 +45,10 @@ def lib():
             "or_intersection": (sz, [vp, sz, vp, sz, vp, i32, sz, C.c_double]),
             "or_orient": (i32, [u32, vp, vp, vp, vp]),
             "or_run_shmem": (i32, [u32, vp, vp, i32, i32, C.POINTER(u64), vp, C.POINTER(OrStats)]),
+            "or_run_shmem_ex": (i32, [u32, vp, vp, i32, i32, i32, C.POINTER(u64), vp, C.POINTER(OrStats)]),
             "or_brute_force": (i32, [u32, vp, vp, C.POINTER(u64), vp]),
+            "or_run_shmem_sampled": (i32, [u32, vp, vp, u32, C.POINTER(u64), C.POINTER(u64),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "or_owner": (i32, [u64, vp, i32]),
             "or_rank_sim": (i32, [u32, vp, vp, i32, vp, i32, i32, i32, C.POINTER(u64), C.POINTER(u64), vp,
                                   C.POINTER(OrStats)]),
@@ -99,15 +102,29 @@ def orient(g):
     return off, heads[: g.nnz // 2]
 
 
-def run_shmem(g, method=MERGE, threads=1, per_vertex=True):
+def run_shmem(g, method=MERGE, threads=1, per_vertex=True, pseudo2core=False):
     """(total, per_vertex or None, stats dict)."""
     tot = C.c_uint64(0)
     pv = np.zeros(g.n, dtype=np.uint64) if per_vertex else None
     st = OrStats()
     off, hd = _u64(g.offsets), _u32(g.heads)
-    lib().or_run_shmem(g.n, _p(off), _p(hd) if g.nnz else None, method, threads, C.byref(tot), _p(pv),
-                       C.byref(st))
+    lib().or_run_shmem_ex(g.n, _p(off), _p(hd) if g.nnz else None, method, threads, 1 if pseudo2core else 0,
+                          C.byref(tot), _p(pv), C.byref(st))
     return int(tot.value), pv, {f: getattr(st, f) for f, _ in OrStats._fields_}
+
+
+def run_shmem_sample(g, stride=16, per_vertex=False):
+    """One-thread timing sample: (sample triangles, None, {prep_s, local_s,
+    sample_edges}) -- preprocessing of the whole graph, local phase over the
+    tails v % stride == 0."""
+    tri, edges = C.c_uint64(0), C.c_uint64(0)
+    ps, ls = C.c_double(0), C.c_double(0)
+    off, hd = _u64(g.offsets), _u32(g.heads)
+    rc = lib().or_run_shmem_sampled(g.n, _p(off), _p(hd) if g.nnz else None, stride, C.byref(tri), C.byref(edges),
+                                    C.byref(ps), C.byref(ls))
+    if rc != 0:
+        raise ValueError("bad stride")
+    return int(tri.value), None, {"prep_s": ps.value, "local_s": ls.value, "sample_edges": int(edges.value)}
 
 
 def brute_force(g):

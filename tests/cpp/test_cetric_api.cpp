@@ -154,6 +154,132 @@ int main(int argc, char** argv) {
         }
         CHECK(threw);
     }
+    {
+        // asymmetric although half of the slots are kept: 0 -> 1 and 1 -> 2
+        bool threw = false;
+        try {
+            katric::DistributedGraph bad({0, 1, 2, 2, 2}, {1, 2});
+            katric::profiling::Statistics stats;
+            katric::run_shmem(ctx, bad, stats, conf);
+        } catch (const katric::Error& e) {
+            threw = e.status() == KT_EINVAL;
+        }
+        CHECK(threw);
+        // a batch graph whose head lies outside it
+        threw = false;
+        try {
+            std::vector<katric::DistributedGraph> bg = {complete(3), katric::DistributedGraph({0, 1, 2}, {1, 2})};
+            katric::count_batch(ctx, bg);
+        } catch (const katric::Error& e) {
+            threw = e.status() == KT_EINVAL;
+        }
+        CHECK(threw);
+    }
+    {
+        // the reference's own arrays: heads as RankEncodedNodeId values with
+        // the rank bits set (0xffff before find_ghost_ranks, or a rank)
+        const auto& G = graphs[2];
+        std::vector<uint64_t> enc(G.head().size());
+        for (size_t i = 0; i < enc.size(); ++i)
+            enc[i] = (uint64_t)G.head()[i] | ((uint64_t)(i % 3 ? 0xffffu : 7u) << katric::graph::NON_RANK_BITS);
+        auto H = katric::DistributedGraph::from_rank_encoded(G.first_out(), enc);
+        katric::profiling::Statistics stats;
+        CHECK(katric::run_shmem(ctx, H, stats, conf) == 45);
+        bool threw = false;
+        try {
+            enc[0] = (uint64_t)G.local_node_count() | (3ull << katric::graph::NON_RANK_BITS);
+            katric::DistributedGraph::from_rank_encoded(G.first_out(), enc);
+        } catch (const katric::Error& e) {
+            threw = e.status() == KT_EINVAL;
+        }
+        CHECK(threw);
+        std::printf("rank-encoded hiv = 45\n");
+    }
+    {
+        // Config knobs: every accepted intersection method gives the same
+        // count; pseudo2core skips the tails with fewer than two out-neighbours
+        const auto& G = graphs[2];
+        for (auto m : {katric::IntersectionMethod::merge, katric::IntersectionMethod::binary_search,
+                       katric::IntersectionMethod::hybrid}) {
+            katric::Config c2;
+            c2.intersection_method = m;
+            katric::profiling::Statistics stats;
+            CHECK(katric::run_shmem(ctx, const_cast<katric::DistributedGraph&>(G), stats, c2) == 45);
+            CHECK(stats.local.skipped_nodes == 0);
+        }
+        katric::Config c3;
+        c3.pseudo2core = true;
+        katric::profiling::Statistics s3;
+        CHECK(katric::run_shmem(ctx, const_cast<katric::DistributedGraph&>(G), s3, c3) == 45);
+        // out-degrees under the (degree, id) order, counted here
+        uint64_t skipped = 0;
+        for (uint64_t v = 0; v < G.local_node_count(); ++v) {
+            uint64_t dplus = 0;
+            for (uint64_t e = G.first_out()[v]; e < G.first_out()[v + 1]; ++e) {
+                const uint32_t u = G.head()[e];
+                dplus += (G.degree(v) < G.degree(u) || (G.degree(v) == G.degree(u) && v < u)) ? 1 : 0;
+            }
+            skipped += dplus < 2 ? 1 : 0;
+        }
+        CHECK(s3.local.skipped_nodes == skipped && skipped > 0);
+        std::printf("pseudo2core hiv: skipped %llu\n", (unsigned long long)s3.local.skipped_nodes);
+        bool threw = false;
+        try {
+            katric::Config c4;
+            c4.intersection_method = katric::IntersectionMethod::binary;
+            katric::profiling::Statistics s4;
+            katric::run_shmem(ctx, const_cast<katric::DistributedGraph&>(G), s4, c4);
+        } catch (const katric::Error& e) {
+            threw = e.status() == KT_EINVAL;
+        }
+        CHECK(threw);
+        threw = false;
+        try {
+            katric::Config c5;
+            c5.num_threads = 0;
+            katric::profiling::Statistics s5;
+            katric::run_shmem(ctx, const_cast<katric::DistributedGraph&>(G), s5, c5);
+        } catch (const katric::Error& e) {
+            threw = e.status() == KT_EINVAL;
+        }
+        CHECK(threw);
+        // the context's own knobs are back
+        kt_params p{};
+        kt_get_params(ctx.get(), &p);
+        CHECK(p.pseudo2core == 0 && p.bsearch_min_deg == 8192);
+    }
+    {
+        // cetric::intersection's hand vectors (tests/intersection_test.cpp:41-52),
+        // both argument orders
+        struct Case {
+            std::vector<uint32_t> a, b, want;
+        } cases[] = {{{0, 1, 3, 6, 9, 478}, {1, 6, 7, 8, 478}, {1, 6, 478}},
+                     {{}, {1, 6, 7, 8, 478}, {}},
+                     {{}, {}, {}},
+                     {{1}, {}, {}},
+                     {{1}, {1}, {1}},
+                     {{1}, {42}, {}},
+                     {{13, 46, 54, 58, 64}, {6, 21, 43, 47, 75}, {}},
+                     {{1, 2, 5, 9, 10}, {1, 2, 4, 5, 6, 7, 8}, {1, 2, 5}},
+                     {{2, 3, 4, 7, 10}, {3, 5, 6}, {3}},
+                     {{1, 2, 3, 4, 5}, {6, 7, 8, 9, 10, 11, 12}, {}}};
+        for (const auto& cs : cases) {
+            for (int swap = 0; swap < 2; ++swap) {
+                std::vector<uint32_t> got;
+                katric::intersection(ctx, swap ? cs.b : cs.a, swap ? cs.a : cs.b,
+                                     [&](uint32_t w) { got.push_back(w); });
+                CHECK(got == cs.want);
+            }
+        }
+        bool threw = false;
+        try {
+            katric::intersection(ctx, std::vector<uint32_t>{3, 2}, std::vector<uint32_t>{2}, [](uint32_t) {});
+        } catch (const katric::Error& e) {
+            threw = e.status() == KT_EINVAL;
+        }
+        CHECK(threw);
+        std::printf("intersection hand vectors: %zu cases\n", sizeof(cases) / sizeof(cases[0]));
+    }
     std::printf(failures ? "FAILED %d\n" : "ALL OK\n", failures);
     return failures ? 1 : 0;
 }

@@ -147,3 +147,36 @@ def test_serve_page_locked_malformed_isolated(ctx):
     good = res[:3] + res[4:]
     assert [r.total for r in good] == alone
     assert res[3].status == K.KT_EINVAL
+
+
+def test_serve_unsorted_under_tight_budget(ctx):
+    # unsorted requests: a batch admitted without the radix-sort scratch is
+    # re-launched with it; when that no longer fits the slot's half of the
+    # budget, its requests run one at a time instead of failing together.
+    # Several budgets around the point where that happens: every result must
+    # equal the oracle, and at least one budget must have isolated a batch.
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(4)
+    graphs = []
+    for s in range(8):
+        g = K.gen_gnm(2000, 16000, seed=40 + s)
+        heads = g.heads.copy()
+        for v in range(g.n):
+            a, b = int(g.offsets[v]), int(g.offsets[v + 1])
+            heads[a:b] = rng.permutation(heads[a:b])
+        graphs.append(K.Graph(g.n, g.offsets, heads))
+    want = [O.run_shmem(g)[:2] for g in graphs]
+    isolated = 0
+    for mib in (3, 4, 5, 6, 8, 12, 64):
+        with K.Context(0) as sctx:
+            srv = KS.Server(sctx, hbm_budget_bytes=mib << 20)
+            pvs = [np.zeros(g.n, np.uint64) for g in graphs]
+            tickets = [srv.submit(g, pv) for g, pv in zip(graphs, pvs)]
+            res = [srv.wait(t) for t in tickets]
+            isolated += srv.stats()["isolation_reruns"]
+            srv.stop()
+        for (t, opv), r, pv in zip(want, res, pvs):
+            assert r.status == 0 and r.total == t, (mib, r.status)
+            np.testing.assert_array_equal(pv, opv)
+    assert isolated > 0

@@ -95,6 +95,15 @@ def test_metis_errors(tmp_path):
         K.read_metis(str(tmp_path / "missing.metis"))
 
 
+def test_metis_oversized_header(tmp_path):
+    # a header promising 10^17 edges must not size an allocation: the file
+    # holds one triangle and reads as such
+    p = tmp_path / "big_m.metis"
+    p.write_text("3 99999999999999999\n2 3\n1 3\n1 2\n")
+    g = K.read_metis(str(p))
+    assert g.n == 3 and g.nnz == 6
+
+
 def test_checksum_sensitive():
     g = K.read_metis(golden("hiv.metis"))
     h = K.Graph(g.n, g.offsets, g.heads.copy())

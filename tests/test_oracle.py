@@ -320,3 +320,47 @@ def test_planted_mistake_is_caught():
     sim = np.array([O.rank_sim(g, P, r, O.CETRIC)[1] for r in range(P)])
     assert sim.sum() == wrong.sum()
     assert not np.array_equal(sim, wrong)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_pseudo2core_counters(seed):
+    """Config::pseudo2core (cetric_edge_iterator.h:239-242): tails with
+    d+(v) < 2 are skipped and counted; counts are unchanged; the work counters
+    lose exactly the skipped tails' terms, recomputed here in numpy."""
+    g = K.gen_rmat(11, 8, seed=seed)
+    t0, pv0, s0 = O.run_shmem(g, threads=3)
+    t1, pv1, s1 = O.run_shmem(g, threads=3, pseudo2core=True)
+    assert t0 == t1
+    np.testing.assert_array_equal(pv0, pv1)
+    ooff, oh = O.orient(g)
+    dp = np.diff(ooff).astype(np.int64)
+    tails = np.repeat(np.arange(g.n), dp)
+    keep = dp[tails] >= 2
+    assert s1["skipped_nodes"] == int(np.sum(dp < 2))
+    assert s1["wedge_checks"] == int(dp[oh][keep].sum())
+    assert s1["intersection_size_local"] == int((dp[tails] + dp[oh])[keep].sum())
+    assert s0["skipped_nodes"] == 0
+    assert s0["wedge_checks"] == int(dp[oh].sum())
+
+
+def test_single_rank_katric_is_one_thread_shmetric():
+    """CeTriC at P = 1: no ghosts, no messages -- the same emits and work
+    counters as shmetric's one-thread local phase."""
+    g = K.gen_rmat(11, 16, seed=6)
+    loc, t3, pv, st = O.rank_sim(g, 1, 0, O.CETRIC)
+    t, pv1, st1 = O.run_shmem(g, threads=1)
+    assert (loc, t3) == (t, 0)
+    np.testing.assert_array_equal(pv, pv1)
+    for k in ("wedge_checks", "intersection_size_local"):
+        assert st[k] == st1[k]
+
+
+def test_sampled_timing_helper():
+    g = K.gen_rmat(10, 16, seed=2)
+    t, _, _ = O.run_shmem(g, per_vertex=False)
+    tri, _, st = O.run_shmem_sample(g, stride=1)
+    assert tri == t and st["sample_edges"] == g.m
+    tri4, _, st4 = O.run_shmem_sample(g, stride=4)
+    ooff, _ = O.orient(g)
+    assert st4["sample_edges"] == int(np.diff(ooff.astype(np.int64))[::4].sum())
+    assert tri4 <= t
