@@ -101,8 +101,9 @@ def test_asymmetric_in_batch(ctx):
         assert e.value.status == K.KT_EINVAL
     # one graph with a surplus of kept slots, another with a shortfall:
     # together the kept count is balanced (a per-union count check passes)
-    surplus = K.Graph(3, np.array([0, 1, 1, 1], np.uint64), np.array([1], np.uint32))   # 0 -> 1 (kept)
-    short = K.Graph(3, np.array([0, 0, 1, 1], np.uint64), np.array([0], np.uint32))     # 1 -> 0 (not kept)
+    # surplus: 1 <-> 2 plus 0 -> 1, kept because (1, 0) < (1, 1); short: 1 -> 0 alone, not kept
+    surplus = K.Graph(3, np.array([0, 1, 2, 3], np.uint64), np.array([1, 2, 1], np.uint32))
+    short = K.Graph(2, np.array([0, 0, 1], np.uint64), np.array([0], np.uint32))
     b = KB.from_graphs([surplus, short])
     assert 2 * sum(kept_slots(x) for x in (surplus, short)) == b.nnz
     with pytest.raises(K.KatricError) as e:
