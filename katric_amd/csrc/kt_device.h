@@ -73,10 +73,17 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 
 // Signed contribution of the slot v -> u to a keyed checksum over unordered
 // pairs: +h({v,u}) when v < u, -h({v,u}) when v > u, so the two slots of an
-// undirected edge cancel.
+// undirected edge cancel.  h is two rounds of 32-bit multiply-xorshift
+// (murmur-style), the second round seeded by the first, so both halves of the
+// 64-bit value depend on both ids and on the key; two quarter-rate multiplies
+// per slot (the orientation pass that computes it is VALU-bound).
 __device__ __forceinline__ uint64_t pair_term(uint32_t v, uint32_t u, uint64_t key) {
-    const uint64_t a = min(v, u), b = max(v, u);
-    const uint64_t h = mix64(((a << 32) | b) ^ key);
+    const uint32_t a = min(v, u), b = max(v, u);
+    uint32_t x = (a ^ (uint32_t)key) * 0x9E3779B1u;
+    x ^= x >> 15;
+    uint32_t y = ((b ^ (uint32_t)(key >> 32)) + x) * 0x85EBCA77u;
+    y ^= y >> 13;
+    const uint64_t h = ((uint64_t)(x ^ __builtin_amdgcn_alignbit(y, y, 11)) << 32) | y;
     return v < u ? h : (0ull - h);
 }
 

@@ -27,13 +27,13 @@ void tile_bounds_launch(uint32_t n, uint64_t nnz, uint64_t tile, const uint64_t*
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz);
 // Validation of a symmetric CSR beside the orientation count pass.  Every
 // slot v -> u adds +h(min, max) (v < u) or -h(min, max) (v > u) of the
-// unordered pair to two 64-bit sums, h being two independently keyed 64-bit
-// mixes; without duplicates in a row, a symmetric CSR sums to zero and an
-// asymmetric one does not, except when both keyed sums collide (probability
-// about 2^-128 for a key drawn at random).  With vbase (G + 1 entries, a
-// disjoint union of G graphs), a head outside its row's graph is an error.
+// unordered pair to a 64-bit sum, h being a keyed 64-bit mix; without
+// duplicates in a row, a symmetric CSR sums to zero and an asymmetric one
+// does not, except when the keyed sum collides (probability about 2^-64 for
+// a key drawn at random).  With vbase (G + 1 entries, a disjoint union of G
+// graphs), a head outside its row's graph is an error.
 struct OrientCheck {
-    uint64_t key[2];               // hash keys
+    uint64_t key[2];               // hash keys (key[0] is used)
     const uint64_t* vbase = nullptr;
     uint32_t G = 0;
 };

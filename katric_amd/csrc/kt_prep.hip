@@ -189,14 +189,10 @@ __device__ __forceinline__ bool outside_graph(const uint64_t* __restrict__ vbase
     return u < vbase[g] || u >= vbase[g + 1];
 }
 
-// Adds a thread's two checksum terms to sym[0..1] (one atomic per wave).
-__device__ __forceinline__ void sym_flush(uint64_t s0, uint64_t s1, uint64_t* __restrict__ sym) {
+// Adds a thread's checksum term to sym[0] (one atomic per wave).
+__device__ __forceinline__ void sym_flush(uint64_t s0, uint64_t* __restrict__ sym) {
     s0 = wave_sum(s0);
-    s1 = wave_sum(s1);
-    if (lane_id() == 0) {
-        if (s0) atomic_add_u64(&sym[0], s0);
-        if (s1) atomic_add_u64(&sym[1], s1);
-    }
+    if (lane_id() == 0 && s0) atomic_add_u64(&sym[0], s0);
 }
 
 struct CheckArgs {
@@ -205,6 +201,7 @@ struct CheckArgs {
     uint32_t G;
 };
 
+template <bool BATCH>
 __global__ void __launch_bounds__(OR_THREADS)
     k_orient_keep(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
                   const uint32_t* __restrict__ deg, const uint8_t* __restrict__ dcls, const uint32_t* __restrict__ tbnd,
@@ -225,9 +222,9 @@ __global__ void __launch_bounds__(OR_THREADS)
     const uint64_t nrows = (uint64_t)hi - lo + 1;
     const bool lds_rows = nrows <= OR_TILE;
     TileGraphs tg{};
-    if (ck.vbase) tg = tile_graphs(ck.vbase, ck.G, lo, hi);
+    if (BATCH) tg = tile_graphs(ck.vbase, ck.G, lo, hi);
     uint32_t flags = 0, cnt = 0;
-    uint64_t s0 = 0, s1 = 0;
+    uint64_t s0 = 0;
     // The 64 slots of a wave are consecutive: the next slot's head and row
     // come from the next lane (the last lane loads them), and kept slots are
     // counted per run of equal rows with one LDS atomic from the run's last lane.
@@ -258,7 +255,7 @@ __global__ void __launch_bounds__(OR_THREADS)
         const uint32_t v = lo + rr;
         const uint32_t u = hu[k];
         if (in) {
-            if (u >= n || (ck.vbase && outside_graph(ck.vbase, tg, v, u))) {
+            if (u >= n || (BATCH && outside_graph(ck.vbase, tg, v, u))) {
                 flags |= KT_ERR_HEAD_RANGE;
             } else {
                 if (u == v) flags |= KT_ERR_SELF_LOOP;
@@ -266,7 +263,6 @@ __global__ void __launch_bounds__(OR_THREADS)
                 // exact degrees, a rare gather)
                 keep = class_before(cv[k], v, cu[k], u, deg);
                 s0 += pair_term(v, u, ck.k0);
-                s1 += pair_term(v, u, ck.k1);
             }
         }
         // next slot in the same row? (shuffles outside any branch: every lane active)
@@ -305,7 +301,7 @@ __global__ void __launch_bounds__(OR_THREADS)
             keep_bits[(tile0 + k * OR_THREADS + (tid & ~63u)) / 64] = bits;
     }
     if (flags) atomicOr(err, flags);
-    sym_flush(s0, s1, sym);
+    sym_flush(s0, sym);
     __syncthreads();
     if (lds_rows) {
         // rows wholly inside the tile get a plain store, the two boundary rows an atomic
@@ -329,7 +325,7 @@ __global__ void __launch_bounds__(OR_THREADS)
 __global__ void k_sym_final(const uint64_t* __restrict__ sym, const uint64_t* __restrict__ kept, uint64_t nnz,
                             uint32_t* __restrict__ err) {
     if (threadIdx.x != 0) return;
-    bool bad = (sym[0] | sym[1]) != 0;
+    bool bad = sym[0] != 0;
     if (kept) bad |= 2 * kept[0] != nnz;
     if (bad) atomicOr(err, KT_ERR_ASYMMETRIC);
 }
@@ -350,7 +346,7 @@ __global__ void __launch_bounds__(OR_THREADS)
     uint32_t lo, hi;
     tile_rows<OR_THREADS, OR_ITEMS>(off, n, tile0, tend, marks, bnd, wmax, lo, hi, tbnd);
     uint32_t flags = 0;
-    uint64_t s0 = 0, s1 = 0;
+    uint64_t s0 = 0;
 #pragma unroll 4
     for (int k = 0; k < OR_ITEMS; ++k) {
         const unsigned i = k * OR_THREADS + tid;
@@ -365,7 +361,6 @@ __global__ void __launch_bounds__(OR_THREADS)
             } else {
                 if (u == v) flags |= KT_ERR_SELF_LOOP;
                 s0 += pair_term(v, u, ck.k0);
-                s1 += pair_term(v, u, ck.k1);
             }
         }
         uint32_t nx = __shfl_down(u, 1, 64);
@@ -383,7 +378,7 @@ __global__ void __launch_bounds__(OR_THREADS)
         }
     }
     if (flags) atomicOr(err, flags);
-    sym_flush(s0, s1, sym);
+    sym_flush(s0, sym);
 }
 
 // Copies the kept heads in slot order: output position = tile base + rank of
@@ -615,8 +610,12 @@ void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const ui
         const CheckArgs ck{chk.key[0], chk.key[1], chk.G ? chk.vbase : nullptr, chk.G};
         fill_u32(w.tbnd, 2 * w.tiles, 0, st);
         k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, w.tbnd);
-        k_orient_keep<<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.dcls, w.tbnd, w.bits, w.dplus,
-                                                               w.tile_cnt, err, ck, w.sym);
+        if (ck.vbase)
+            k_orient_keep<true><<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.dcls, w.tbnd,
+                                                                         w.bits, w.dplus, w.tile_cnt, err, ck, w.sym);
+        else
+            k_orient_keep<false><<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.dcls, w.tbnd,
+                                                                          w.bits, w.dplus, w.tile_cnt, err, ck, w.sym);
         scan_u64(w.tile_cnt, w.tile_off, w.tiles, w.stmp, st);
     } else {
         fill_u64(w.tile_off, 1, 0, st);
