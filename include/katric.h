@@ -185,6 +185,20 @@ int kt_intersect(kt_ctx* ctx, uint64_t num_pairs, const uint64_t* a_offsets, con
                  const uint64_t* b_offsets, const uint32_t* b_ids, uint32_t flags, uint64_t* counts,
                  uint32_t* matches, uint64_t capacity, uint64_t* total_out);
 
+/* kt_count without any allocation: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid it).  No
+ * counterpart in the reference.  The device workspace is a fixed array of
+ * KT_NOALLOC_SCRATCH_BYTES in the library, one per GPU, held until the
+ * process ends; a graph whose workspace exceeds it (about 24 bytes per vertex
+ * plus 2.2 bytes per CSR entry for sorted lists, plus 16 bytes per edge for
+ * unsorted ones) is KT_ENOMEM.  offsets / heads are device pointers,
+ * per_vertex_out a device pointer (nullable), *total_out is written on the
+ * host.  Calls on one GPU share the array: this library serialises them
+ * within a process; calls that overlap on one GPU through other means race. */
+#define KT_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int kt_count_noalloc(kt_ctx* ctx, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint64_t* total_out,
+                     uint64_t* per_vertex_out);
+
 /* Batched count of many independent graphs given as one disjoint-union CSR:
  * graph g owns the global ids [vertex_base[g], vertex_base[g+1]) (host array
  * of num_graphs+1), offsets has vertex_base[num_graphs]+1 entries and heads

@@ -80,6 +80,7 @@ SIGNATURES = {
     "kt_list_triangles": (i32, [vp, u32, vp, vp, u32, vp, u64, p_u64]),
     "kt_orient": (i32, [vp, u32, vp, vp, u32, vp, vp]),
     "kt_intersect": (i32, [vp, u64, vp, vp, vp, vp, u32, vp, vp, u64, p_u64]),
+    "kt_count_noalloc": (i32, [vp, u32, vp, vp, p_u64, vp]),
     # katric_graphio.h
     "kt_host_graph_free": (None, [C.POINTER(KtHostGraph)]),
     "kt_gen_gnm": (i32, [u32, u64, u64, C.POINTER(KtHostGraph)]),

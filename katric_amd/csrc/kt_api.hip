@@ -193,6 +193,9 @@ extern "C" int kt_create(kt_ctx** out, int device, uint64_t hbm_budget_bytes) {
         return KT_EHIP;
     }
     random_keys(c->sym_key);
+    // the no-allocation entry's workspace address on this device (retried by
+    // that entry if it fails here)
+    (void)noalloc_init(device);
     c->prm.wave_max_deg = 128;
     c->prm.group_max_deg = 2048;
     c->prm.team_chunk = 8;
@@ -368,16 +371,32 @@ int upload_and_count(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_o
 // Builds the oriented CSR in B.ooff / B.oheads.  The count pass is checked on
 // the host (one stream sync) before anything is written, so malformed or
 // asymmetric input never reaches the write pass or the intersection.
+// The device workspace of a plain call: the context's grow-only arena, or a
+// fixed region given by the caller (never grown: a call that does not fit
+// fails with KT_ENOMEM).
+struct Workspace {
+    void* fixed_base = nullptr;
+    uint64_t fixed_cap = 0;
+    void* base(const kt_ctx* c) const { return fixed_base ? fixed_base : c->arena; }
+    uint64_t cap(const kt_ctx* c) const { return fixed_base ? fixed_cap : c->arena_bytes; }
+    int ensure(kt_ctx* c, uint64_t need) const {
+        if (!fixed_base) return c->reserve(need);
+        if (need <= fixed_cap) return KT_OK;
+        return c->fail(KT_ENOMEM, "graph needs %llu bytes of workspace, the fixed workspace holds %llu",
+                       (unsigned long long)need, (unsigned long long)fixed_cap);
+    }
+};
+
 int run_orientation(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_off, const uint32_t* user_heads,
-                    bool host_in, bool pv_dev_needed, PlainBufs& B, bool& unsorted) {
+                    bool host_in, bool pv_dev_needed, PlainBufs& B, bool& unsorted, const Workspace& ws = {}) {
     hipStream_t st = c->stream;
     unsorted = false;
     uint64_t need = plain_layout(nullptr, n, nnz, host_in, pv_dev_needed, false, B);
     trace("orientation: n=%u nnz=%llu workspace=%llu", n, (unsigned long long)nnz, (unsigned long long)need);
-    int rc = c->reserve(need);
+    int rc = ws.ensure(c, need);
     if (rc) return rc;
-    plain_layout(c->arena, n, nnz, host_in, pv_dev_needed, false, B);
-    trace("arena=%p bytes=%llu", c->arena, (unsigned long long)c->arena_bytes);
+    plain_layout(ws.base(c), n, nnz, host_in, pv_dev_needed, false, B);
+    trace("workspace=%p bytes=%llu", ws.base(c), (unsigned long long)ws.cap(c));
     (void)hipEventRecord(c->ev[EV_START], st);
     rc = upload_and_count(c, n, nnz, user_off, user_heads, host_in, B);
     if (rc) return rc;
@@ -400,15 +419,15 @@ int run_orientation(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_of
     if (flags & KT_FLAG_UNSORTED) {
         unsorted = true;
         need = plain_layout(nullptr, n, nnz, host_in, pv_dev_needed, true, B);
-        if (need > c->arena_bytes) {
+        if (need > ws.cap(c)) {
             // growing the arena drops its contents: upload and count again
-            rc = c->reserve(need);
+            rc = ws.ensure(c, need);
             if (rc) return rc;
-            plain_layout(c->arena, n, nnz, host_in, pv_dev_needed, true, B);
+            plain_layout(ws.base(c), n, nnz, host_in, pv_dev_needed, true, B);
             rc = upload_and_count(c, n, nnz, user_off, user_heads, host_in, B);
             if (rc) return rc;
         } else {
-            plain_layout(c->arena, n, nnz, host_in, pv_dev_needed, true, B);
+            plain_layout(ws.base(c), n, nnz, host_in, pv_dev_needed, true, B);
             if (!host_in) {
                 B.off = user_off;
                 B.heads = user_heads;
@@ -453,12 +472,14 @@ int read_nnz(kt_ctx* c, uint32_t n, const uint64_t* offsets, uint32_t flags, uin
 
 }  // namespace
 
-extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
-                        uint64_t* total_out, uint64_t* per_vertex_out) {
-    if (!c) return KT_EINVAL;
-    std::lock_guard<std::mutex> lk(c->mu);
-    int rc = c->begin_call(true);
-    if (rc) return rc;
+// The body of kt_count on a given workspace (the caller holds c->mu and has
+// begun the call).
+int kt::count_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+                     uint64_t* total_out, uint64_t* per_vertex_out, void* fixed_ws, uint64_t fixed_ws_bytes) {
+    Workspace ws;
+    ws.fixed_base = fixed_ws;
+    ws.fixed_cap = fixed_ws_bytes;
+    int rc;
     if (!total_out || !offsets || (n > 0 && !heads && !(flags & KT_DEVICE_INPUT)))
         return c->fail(KT_EINVAL, "null argument");
     *total_out = 0;
@@ -471,7 +492,7 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     const bool pv_dev = want_pv && (flags & KT_DEVICE_OUTPUT);
     PlainBufs B;
     bool unsorted;
-    rc = run_orientation(c, n, nnz, offsets, heads, host_in, want_pv && !pv_dev, B, unsorted);
+    rc = run_orientation(c, n, nnz, offsets, heads, host_in, want_pv && !pv_dev, B, unsorted, ws);
     if (rc) return rc;
     hipStream_t st = c->stream;
     (void)hipEventRecord(c->ev[EV_PREP], st);
@@ -521,6 +542,15 @@ extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const ui
     c->timing.d2h_ms = c->elapsed(EV_COUNT, EV_END);
     c->timing.total_ms = c->elapsed(EV_START, EV_END);
     return KT_OK;
+}
+
+extern "C" int kt_count(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+                        uint64_t* total_out, uint64_t* per_vertex_out) {
+    if (!c) return KT_EINVAL;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = c->begin_call(true);
+    if (rc) return rc;
+    return count_locked(c, n, offsets, heads, flags, total_out, per_vertex_out, nullptr, 0);
 }
 
 extern "C" int kt_list_triangles(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads,

@@ -80,6 +80,14 @@ namespace kt {
 uint64_t* tc_stats_ptr(uint64_t* ws);
 // kt_count_shard without taking the context lock (kt_shard.hip)
 int count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out);
+// kt_count without taking the context lock; with fixed_ws the device
+// workspace is that region (fixed_ws_bytes long) instead of the context's
+// arena, and nothing is allocated (kt_api.hip)
+int count_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint32_t* heads, uint32_t flags,
+                 uint64_t* total_out, uint64_t* per_vertex_out, void* fixed_ws, uint64_t fixed_ws_bytes);
+// Device address of this device's kt_count_noalloc workspace, looked up once
+// per device with the device current (kt_noalloc.hip).
+hipError_t noalloc_init(int device);
 // KATRIC_TRACE=1: one stderr line per host-side pipeline step
 bool trace_enabled();
 void trace(const char* fmt, ...);
