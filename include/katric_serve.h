@@ -23,8 +23,9 @@ extern "C" {
 typedef struct {
     uint64_t hbm_budget_bytes;  /* device bytes for both batch slots (0: the context budget, else half the free memory) */
     uint32_t max_batch_graphs;  /* 0: 4096 */
-    uint32_t reserved;
-    uint64_t max_batch_edges;   /* undirected edges per batch (0: 2^28) */
+    uint32_t large_request_edges; /* requests with more undirected edges run alone, at most one such batch at a
+                                     time, and smaller requests behind them may go ahead (0: max_batch_edges / 4) */
+    uint64_t max_batch_edges;   /* undirected edges per batch (0: 2^25) */
 } kt_serve_config;
 
 typedef struct {

@@ -43,6 +43,7 @@ struct GatherDesc {
     uint64_t pad;
 };
 constexpr uint32_t GATHER_CHUNK = 4096;  // ids per chunk
+constexpr size_t SERVE_LOOKAHEAD = 4096; // queued requests scanned when forming a batch
 
 __host__ __device__ inline uint64_t gather_chunks(uint64_t n, uint64_t nnz) {
     return (n + GATHER_CHUNK - 1) / GATHER_CHUNK + (nnz + GATHER_CHUNK - 1) / GATHER_CHUNK;
@@ -111,6 +112,7 @@ struct ServeSlot {
     std::vector<ServeReq*> reqs;
     bool busy = false;
     bool sorted = false;
+    bool large = false;  // the batch is one large request
     uint64_t batch_id = 0;
     uint64_t N = 0, nnz = 0;
 };
@@ -122,6 +124,7 @@ struct ServeState {
     TcParams prm{};
     uint64_t sym_key[2] = {0, 0};
     uint64_t slot_cap = 0;
+    uint64_t large_edges = 0;  // requests above this many edges run alone
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -335,23 +338,49 @@ static void worker_main(ServeState* S) {
             S->cv_done.notify_all();
         }
         if (free_slot && !S->pending.empty()) {
-            // form the next batch: FIFO while it fits the slot and the limits
+            // Form the next batch.  A large request (more than large_edges
+            // edges) or an isolated one runs alone, and at most one slot runs a
+            // large request at a time: while the other slot holds one, a large
+            // request at the queue front waits and the small requests behind
+            // it form this slot's batch, so that they are not held up behind
+            // it.  Small requests are taken in arrival order while the batch
+            // fits the slot and the limits.
+            ServeSlot& other = (free_slot == &S->slots[0]) ? S->slots[1] : S->slots[0];
+            const bool other_large = other.busy && other.large;
             std::vector<ServeReq*> batch;
+            bool blocked_large = false;
             uint64_t N = 0, nnz = 0;
-            while (!S->pending.empty()) {
-                ServeReq* r = S->pending.front();
+            for (size_t i = 0; i < S->pending.size() && i < SERVE_LOOKAHEAD;) {
+                ServeReq* r = S->pending[i];
                 if (r->cancelled) {
-                    S->pending.pop_front();
+                    S->pending.erase(S->pending.begin() + (ptrdiff_t)i);
                     r->status = KT_ECANCELED;
                     r->done = true;
+                    S->cv_done.notify_all();
                     continue;
                 }
-                if (!batch.empty() && (r->isolate || batch.front()->isolate)) break;
+                const bool large = r->nnz / 2 > S->large_edges;
+                if (large || r->isolate) {
+                    if (batch.empty() && !(large && other_large)) {
+                        S->pending.erase(S->pending.begin() + (ptrdiff_t)i);
+                        if (req_bytes(r) > S->slot_cap) {
+                            r->status = KT_ENOMEM;
+                            r->done = true;
+                            S->cv_done.notify_all();
+                            continue;
+                        }
+                        batch.push_back(r);
+                        blocked_large = large;
+                        break;
+                    }
+                    ++i;  // keeps its place; smaller requests behind it go ahead
+                    continue;
+                }
                 const uint64_t bytes = slot_bytes((uint32_t)batch.size() + 1, N + r->n, nnz + r->nnz, false);
                 const bool limits = batch.size() + 1 <= S->cfg.max_batch_graphs &&
                                     (nnz + r->nnz) / 2 <= S->cfg.max_batch_edges;
                 if (!batch.empty() && (bytes > S->slot_cap || !limits)) break;
-                S->pending.pop_front();
+                S->pending.erase(S->pending.begin() + (ptrdiff_t)i);
                 if (bytes > S->slot_cap) {  // alone it does not fit the budget share
                     r->status = KT_ENOMEM;
                     r->done = true;
@@ -362,14 +391,17 @@ static void worker_main(ServeState* S) {
                 N += r->n;
                 nnz += r->nnz;
             }
-            if (batch.empty()) continue;
-            free_slot->reqs = batch;
-            free_slot->batch_id = S->next_batch++;
-            lk.unlock();
-            int rc = launch(S, *free_slot, false);
-            lk.lock();
-            if (rc != KT_OK) launch_failed(S, *free_slot, rc);
-            continue;
+            if (!batch.empty()) {
+                free_slot->reqs = batch;
+                free_slot->large = blocked_large;
+                free_slot->batch_id = S->next_batch++;
+                lk.unlock();
+                int rc = launch(S, *free_slot, false);
+                lk.lock();
+                if (rc != KT_OK) launch_failed(S, *free_slot, rc);
+                continue;
+            }
+            // only a large request waits for the other slot's large one: poll below
         }
         bool any_busy = S->slots[0].busy || S->slots[1].busy;
         if (S->stop && S->pending.empty() && !any_busy) break;
@@ -431,7 +463,8 @@ extern "C" int kt_serve_start(kt_ctx* c, const kt_serve_config* cfg) {
     ServeState* S = new ServeState();
     if (cfg) S->cfg = *cfg;
     if (S->cfg.max_batch_graphs == 0) S->cfg.max_batch_graphs = 4096;
-    if (S->cfg.max_batch_edges == 0) S->cfg.max_batch_edges = 256ull << 20;
+    if (S->cfg.max_batch_edges == 0) S->cfg.max_batch_edges = 1ull << 25;
+    S->large_edges = S->cfg.large_request_edges ? S->cfg.large_request_edges : S->cfg.max_batch_edges / 4;
     uint64_t budget = S->cfg.hbm_budget_bytes ? S->cfg.hbm_budget_bytes : c->budget;
     if (budget == 0) {
         size_t fr = 0, tot = 0;

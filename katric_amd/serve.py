@@ -14,7 +14,8 @@ u32, u64, vp = C.c_uint32, C.c_uint64, C.c_void_p
 
 
 class KtServeConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", u64), ("max_batch_graphs", u32), ("reserved", u32), ("max_batch_edges", u64)]
+    _fields_ = [("hbm_budget_bytes", u64), ("max_batch_graphs", u32), ("large_request_edges", u32),
+                ("max_batch_edges", u64)]
 
 
 class KtServeResult(C.Structure):
@@ -85,9 +86,10 @@ def pinned_graph(g: Graph):
 class Server:
     """Continuous-batching server on a Context's device."""
 
-    def __init__(self, ctx, hbm_budget_bytes: int = 0, max_batch_graphs: int = 0, max_batch_edges: int = 0):
+    def __init__(self, ctx, hbm_budget_bytes: int = 0, max_batch_graphs: int = 0, max_batch_edges: int = 0,
+                 large_request_edges: int = 0):
         self.ctx = ctx
-        cfg = KtServeConfig(hbm_budget_bytes, max_batch_graphs, 0, max_batch_edges)
+        cfg = KtServeConfig(hbm_budget_bytes, max_batch_graphs, large_request_edges, max_batch_edges)
         ctx.check(lib().kt_serve_start(ctx.handle, C.byref(cfg)), "kt_serve_start")
         self._keep: Dict[int, tuple] = {}
 

@@ -180,3 +180,27 @@ def test_serve_unsorted_under_tight_budget(ctx):
             assert r.status == 0 and r.total == t, (mib, r.status)
             np.testing.assert_array_equal(pv, opv)
     assert isolated > 0
+
+
+def test_serve_large_requests_run_alone(ctx):
+    # requests above large_request_edges run in batches of their own (one at
+    # a time); the small ones around them may go ahead; every result equals
+    # the one-graph count
+    graphs = mix(50, 9)
+    alone = [ctx.count(g) for g in graphs]
+    big = 30000
+    with K.Context(0) as sctx:
+        srv = KS.Server(sctx, large_request_edges=big)
+        pvs = [np.zeros(g.n, np.uint64) for g in graphs]
+        tickets = [srv.submit(g, pv) for g, pv in zip(graphs, pvs)]
+        res = [srv.wait(t) for t in tickets]
+        srv.stop()
+    n_large = 0
+    for g, (t, pv), r, p in zip(graphs, alone, res, pvs):
+        assert r.status == 0 and r.total == t
+        np.testing.assert_array_equal(p, pv)
+        if g.m > big:
+            n_large += 1
+            assert r.batch_graphs == 1
+    assert 0 < n_large < len(graphs)
+    assert max(r.batch_graphs for r in res) > 1
