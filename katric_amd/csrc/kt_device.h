@@ -101,6 +101,13 @@ __device__ __forceinline__ uint32_t graph_of(const uint64_t* __restrict__ vbase,
     return glo;
 }
 
+// ORs flag bits into a word many threads report to, skipping the atomic when
+// the bits are set already (atomics on one word serialise in L2).
+__device__ __forceinline__ void or_flags(uint32_t* __restrict__ err, uint32_t flags) {
+    if (flags && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & flags) != flags)
+        atomicOr(err, flags);
+}
+
 __device__ __forceinline__ unsigned long long atomic_add_u64(uint64_t* p, uint64_t v) {
     return atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }

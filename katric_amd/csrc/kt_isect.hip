@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(IS_THREADS)
         }
         if (!WRITE && lane == 0) counts[p] = hits;
     }
-    if (flags) atomicOr(err, flags);
+    or_flags(err, flags);
 }
 
 }  // namespace kt

@@ -189,10 +189,17 @@ __device__ __forceinline__ bool outside_graph(const uint64_t* __restrict__ vbase
     return u < vbase[g] || u >= vbase[g + 1];
 }
 
-// Adds a thread's checksum term to sym[0] (one atomic per wave).
-__device__ __forceinline__ void sym_flush(uint64_t s0, uint64_t* __restrict__ sym) {
-    s0 = wave_sum(s0);
-    if (lane_id() == 0 && s0) atomic_add_u64(&sym[0], s0);
+// Sets flag bits in the global error word with one atomic per block, and
+// only when some bit is not set yet: every wave ORing into the same word
+// serialises the whole grid in L2 (the unsorted flag is set by nearly every
+// wave of an unsorted input).
+__device__ __forceinline__ void flags_flush(uint32_t flags, uint32_t* lds_flags, uint32_t* __restrict__ err) {
+    if (flags) atomicOr(lds_flags, flags);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t f = *lds_flags;
+        if (f && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & f) != f) atomicOr(err, f);
+    }
 }
 
 struct CheckArgs {
@@ -207,16 +214,18 @@ __global__ void __launch_bounds__(OR_THREADS)
                   const uint32_t* __restrict__ deg, const uint8_t* __restrict__ dcls, const uint32_t* __restrict__ tbnd,
                   uint64_t* __restrict__ keep_bits,
                   uint64_t* __restrict__ dplus, uint64_t* __restrict__ tile_cnt, uint32_t* __restrict__ err,
-                  CheckArgs ck, uint64_t* __restrict__ sym) {
+                  CheckArgs ck, uint64_t* __restrict__ sym_part) {
     __shared__ uint32_t marks[OR_TILE];
     __shared__ uint32_t rowcnt[OR_TILE];
     __shared__ uint32_t wmax[OR_THREADS / KT_WAVE];
     __shared__ uint32_t bnd[2];
     __shared__ uint64_t lw[OR_THREADS / KT_WAVE];
+    __shared__ uint32_t bflags;
     const uint64_t tile0 = (uint64_t)blockIdx.x * OR_TILE;
     const uint64_t tend = min(tile0 + OR_TILE, nnz);
     const unsigned tid = threadIdx.x;
     for (int i = tid; i < OR_TILE; i += OR_THREADS) rowcnt[i] = 0;
+    if (tid == 0) bflags = 0;
     uint32_t lo, hi;
     tile_rows<OR_THREADS, OR_ITEMS>(off, n, tile0, tend, marks, bnd, wmax, lo, hi, tbnd);
     const uint64_t nrows = (uint64_t)hi - lo + 1;
@@ -224,7 +233,6 @@ __global__ void __launch_bounds__(OR_THREADS)
     TileGraphs tg{};
     if (BATCH) tg = tile_graphs(ck.vbase, ck.G, lo, hi);
     uint32_t flags = 0, cnt = 0;
-    uint64_t s0 = 0;
     // The 64 slots of a wave are consecutive: the next slot's head and row
     // come from the next lane (the last lane loads them), and kept slots are
     // counted per run of equal rows with one LDS atomic from the run's last lane.
@@ -245,6 +253,14 @@ __global__ void __launch_bounds__(OR_THREADS)
         cu[k] = (in && hu[k] < n) ? dcls[hu[k]] : 0u;
         cv[k] = in ? dcls[lo + marks[i]] : 0u;
     }
+    // the pair checksum while the class gathers are in flight: heads and
+    // rows are at hand, and the VALU work hides behind the gathers' latency
+    uint64_t s0 = 0;
+#pragma unroll
+    for (int k = 0; k < OR_ITEMS; ++k) {
+        const unsigned i = k * OR_THREADS + tid;
+        if (tile0 + i < tend && hu[k] < n) s0 += pair_term(lo + marks[i], hu[k], ck.k0);
+    }
 #pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
         const unsigned i = k * OR_THREADS + tid;
@@ -262,7 +278,6 @@ __global__ void __launch_bounds__(OR_THREADS)
                 // classes decide unless they are equal above 127 (then the
                 // exact degrees, a rare gather)
                 keep = class_before(cv[k], v, cu[k], u, deg);
-                s0 += pair_term(v, u, ck.k0);
             }
         }
         // next slot in the same row? (shuffles outside any branch: every lane active)
@@ -300,9 +315,7 @@ __global__ void __launch_bounds__(OR_THREADS)
         if (lane == 0 && tile0 + k * OR_THREADS + (tid & ~63u) < tend)
             keep_bits[(tile0 + k * OR_THREADS + (tid & ~63u)) / 64] = bits;
     }
-    if (flags) atomicOr(err, flags);
-    sym_flush(s0, sym);
-    __syncthreads();
+    flags_flush(flags, &bflags, err);
     if (lds_rows) {
         // rows wholly inside the tile get a plain store, the two boundary rows an atomic
         for (uint64_t rr = tid; rr < nrows; rr += OR_THREADS) {
@@ -315,17 +328,28 @@ __global__ void __launch_bounds__(OR_THREADS)
             }
         }
     }
-    uint64_t tot;
+    uint64_t tot, stot;
     block_excl_sum(cnt, lw, &tot);
-    if (tid == 0) tile_cnt[blockIdx.x] = tot;
+    block_excl_sum(s0, lw, &stot);
+    if (tid == 0) {
+        tile_cnt[blockIdx.x] = tot;
+        sym_part[blockIdx.x] = stot;  // the tile's checksum, summed by k_sym_final
+    }
 }
 
-// Folds the pair checksums and the kept count into the error flags: a
-// symmetric CSR keeps exactly half of its slots and its sums are zero.
-__global__ void k_sym_final(const uint64_t* __restrict__ sym, const uint64_t* __restrict__ kept, uint64_t nnz,
-                            uint32_t* __restrict__ err) {
+// Folds the per-tile pair checksums (tiles entries) and the kept count into
+// the error flags: a symmetric CSR keeps exactly half of its slots and its
+// checksum is zero.  One 256-thread block.
+__global__ void __launch_bounds__(256) k_sym_final(const uint64_t* __restrict__ sym_part, uint64_t tiles,
+                                                   const uint64_t* __restrict__ kept, uint64_t nnz,
+                                                   uint32_t* __restrict__ err) {
+    __shared__ uint64_t lw[256 / KT_WAVE];
+    uint64_t s = 0;
+    for (uint64_t i = threadIdx.x; i < tiles; i += 256) s += sym_part[i];
+    uint64_t tot;
+    block_excl_sum(s, lw, &tot);
     if (threadIdx.x != 0) return;
-    bool bad = sym[0] != 0;
+    bool bad = tot != 0;
     if (kept) bad |= 2 * kept[0] != nnz;
     if (bad) atomicOr(err, KT_ERR_ASYMMETRIC);
 }
@@ -336,13 +360,16 @@ __global__ void k_sym_final(const uint64_t* __restrict__ sym, const uint64_t* __
 __global__ void __launch_bounds__(OR_THREADS)
     k_sym_tiles(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
                 const uint32_t* __restrict__ tbnd, uint32_t* __restrict__ err, CheckArgs ck,
-                uint64_t* __restrict__ sym) {
+                uint64_t* __restrict__ sym_part) {
     __shared__ uint32_t marks[OR_TILE];
     __shared__ uint32_t wmax[OR_THREADS / KT_WAVE];
     __shared__ uint32_t bnd[2];
+    __shared__ uint64_t lw[OR_THREADS / KT_WAVE];
+    __shared__ uint32_t bflags;
     const uint64_t tile0 = (uint64_t)blockIdx.x * OR_TILE;
     const uint64_t tend = min(tile0 + OR_TILE, nnz);
     const unsigned tid = threadIdx.x, lane = lane_id();
+    if (tid == 0) bflags = 0;
     uint32_t lo, hi;
     tile_rows<OR_THREADS, OR_ITEMS>(off, n, tile0, tend, marks, bnd, wmax, lo, hi, tbnd);
     uint32_t flags = 0;
@@ -377,8 +404,10 @@ __global__ void __launch_bounds__(OR_THREADS)
                 flags |= KT_FLAG_UNSORTED;
         }
     }
-    if (flags) atomicOr(err, flags);
-    sym_flush(s0, sym);
+    flags_flush(flags, &bflags, err);
+    uint64_t stot;
+    block_excl_sum(s0, lw, &stot);
+    if (tid == 0) sym_part[blockIdx.x] = stot;
 }
 
 // Copies the kept heads in slot order: output position = tile base + rank of
@@ -558,7 +587,7 @@ void radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t len, int bits, uint6
 // ---------------------------------------------------------------------------
 struct OrientWs {
     uint64_t tiles;
-    uint64_t* sym;       // 2: pair checksums
+    uint64_t* sym;       // tiles: per-tile pair checksums
     uint64_t* tile_cnt;  // tiles + 1
     uint64_t* tile_off;  // tiles + 1 (tile_off[tiles] = kept total)
     uint64_t* bits;      // ceil(nnz / 64) + 1
@@ -572,7 +601,7 @@ static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
     OrientWs w;
     w.tiles = (nnz + OR_TILE - 1) / OR_TILE;
     w.sym = ws;
-    w.tile_cnt = ws + 2;
+    w.tile_cnt = ws + w.tiles + 1;
     w.tile_off = w.tile_cnt + w.tiles + 1;
     w.bits = w.tile_off + w.tiles + 1;
     w.dplus = w.bits + (nnz + 63) / 64 + 1;
@@ -585,7 +614,7 @@ static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
 
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz) {
     uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
-    return 2 + 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 + tiles +
+    return (tiles + 1) + 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 + tiles +
            std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1)) + ((uint64_t)n + 7) / 8;
 }
 
@@ -605,7 +634,6 @@ void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const ui
     OrientWs w = orient_ws(n, nnz, ws);
     k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, w.dcls, err);
     fill_u64(w.dplus, (uint64_t)n + 1, 0, st);
-    fill_u64(w.sym, 2, 0, st);
     if (w.tiles > 0) {
         const CheckArgs ck{chk.key[0], chk.key[1], chk.G ? chk.vbase : nullptr, chk.G};
         fill_u32(w.tbnd, 2 * w.tiles, 0, st);
@@ -621,28 +649,27 @@ void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const ui
         fill_u64(w.tile_off, 1, 0, st);
     }
     scan_u64(w.dplus, ooff, n, w.stmp, st);
-    k_sym_final<<<1, 64, 0, st>>>(w.sym, ooff + n, nnz, err);
+    k_sym_final<<<1, 256, 0, st>>>(w.sym, w.tiles, ooff + n, nnz, err);
 }
 
 uint64_t sym_ws_words(uint32_t n, uint64_t nnz) {
     (void)n;
     const uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
-    return 2 + tiles + 1;
+    return 2 * (tiles + 1);  // per-tile checksums, then the tiles' row bounds
 }
 
 void sym_check_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint64_t* ws,
                       uint32_t* err, const OrientCheck& chk, hipStream_t st) {
     const uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
     uint64_t* sym = ws;
-    uint32_t* tbnd = reinterpret_cast<uint32_t*>(ws + 2);
-    fill_u64(sym, 2, 0, st);
+    uint32_t* tbnd = reinterpret_cast<uint32_t*>(ws + tiles + 1);
     if (tiles > 0 && n > 0) {
         const CheckArgs ck{chk.key[0], chk.key[1], nullptr, 0};
         fill_u32(tbnd, 2 * tiles, 0, st);
         k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, tbnd);
         k_sym_tiles<<<(unsigned)tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, tbnd, err, ck, sym);
     }
-    k_sym_final<<<1, 64, 0, st>>>(sym, nullptr, nnz, err);
+    k_sym_final<<<1, 256, 0, st>>>(sym, (tiles > 0 && n > 0) ? tiles : 0, nullptr, nnz, err);
 }
 
 void random_keys(uint64_t key[2]) {

@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(SH_THREADS)
         pm[k] = __ballot(out_e || in_e);
         count += (uint32_t)__popcll(pm[k]);
     }
-    if (flags) atomicOr(err, flags);
+    or_flags(err, flags);
     if (lane_id() == 0) wtot[wave] = count;
     __syncthreads();
     if (tid == 0) {
@@ -308,7 +308,7 @@ __global__ void k_msg_count(ShardMap M, uint32_t nmsg, uint64_t mtot, const uint
             nown[i] = own;
         }
     }
-    if (flags) atomicOr(err, flags);
+    or_flags(err, flags);
 }
 
 // Per message: owned entries then ghost entries, each in message order.

@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(PD_THREADS)
             if (before(deg[u], u, deg[x], x)) set_bit(sbits, u);
         }
     }
-    if (flags) atomicOr(err, flags);
+    or_flags(err, flags);
 }
 
 __global__ void k_pd_popc(uint64_t words, const uint32_t* __restrict__ gbits, const uint32_t* __restrict__ sbits,
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(PD_THREADS)
         }
         if (lane_id() == 0) wcnt[k * NW + wave] = (uint32_t)__popcll(bits[k]);
     }
-    if (!EMIT && flags) atomicOr(err, flags);
+    if (!EMIT) or_flags(err, flags);
     __syncthreads();
     if (!EMIT) {
         if (tid == 0) {
