@@ -53,7 +53,7 @@ MEASURED_TRAFFIC = {("plain", 22, 16, 1): 120.69e9, ("shard-cetric", 26, 16, 1, 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=["plain", "batch", "serve", "shard", "copy"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "unsorted", "batch", "serve", "shard", "copy"], default="plain")
     ap.add_argument("--requests", type=int, default=20000, help="serve mode: requests in the Poisson stream")
     ap.add_argument("--rate", type=float, default=0.0, help="serve mode: arrivals/s (0: 70%% of saturation)")
     ap.add_argument("--small-mix", action="store_true",
@@ -127,6 +127,8 @@ def main():
         return run_serve(args)
     if args.mode == "copy":
         return run_copy(args)
+    if args.mode == "unsorted":
+        return run_unsorted(args)
     if args.mode == "shard":
         if args.scale == 22 and "--scale" not in sys.argv:
             args.scale = 26
@@ -585,8 +587,10 @@ def design_bytes(wedge_checks, m, n):
 def unsorted_leg(args, ctx, g, total):
     """The same graph with every adjacency list in descending order, resident
     in HBM: orientation finds the lists unsorted and sorts the oriented lists
-    with the device radix sort -- the path the reference always pays
-    (sort_neighborhoods, distributed_graph.h:577-593)."""
+    on the device (each list in place by a lane, a wave or a workgroup; a
+    list above 4096 ids would send the graph to the LSD radix sort) -- the
+    path the reference always pays (sort_neighborhoods,
+    distributed_graph.h:577-593)."""
     deg = np.diff(g.offsets.astype(np.int64))
     rows = np.repeat(np.arange(g.n, dtype=np.int64), deg)
     pos = g.offsets[rows].astype(np.int64) + g.offsets[rows + 1].astype(np.int64) - 1 - np.arange(g.nnz)
@@ -614,9 +618,28 @@ def unsorted_leg(args, ctx, g, total):
     d_pv.free()
     return {"ms_per_step": ms, "edges_per_s": g.m / (ms * 1e-3), "prep_ms": float(np.mean(prep)),
             "device_ms": float(np.mean(dev)), "steps": steps,
-            "workload": "the same RMAT-22 CSR with every list in descending order, resident in HBM; orientation + "
-                        "device LSD radix sort of the oriented lists + intersection",
+            "workload": f"the same RMAT-{args.scale} CSR with every list in descending order, resident in HBM; "
+                        "orientation + segmented device sort of the oriented lists + intersection",
             "checked": "total equals the sorted input's"}
+
+
+def run_unsorted(args):
+    """RMAT-22 with every adjacency list in descending order (the unsorted leg
+    of the default run, on its own)."""
+    import katric_amd as K
+    from katric_amd import tune as T
+
+    g = K.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
+    ctx = K.Context(0)
+    knobs = apply_params(ctx, args, T.graph_key(g.n, g.offsets))
+    total = ctx.count(g, per_vertex=False)[0]
+    leg = unsorted_leg(args, ctx, g, total)
+    print(json.dumps({
+        "metric": f"edges/s (exact triangle count, RMAT-{args.scale} ef{args.edge_factor}, unsorted lists)",
+        "value": leg["edges_per_s"], "unit": "edges/s", "n_gpus": 1, "steps": leg["steps"], "warmup": 2,
+        "ms_per_step": leg["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 ids / u64 counts", "data": "synthetic (seeded RMAT generator, permuted ids, lists reversed)",
+        "config": {"params": knobs, **leg}, "cpu_baseline": None}), flush=True)
 
 
 def run_plain(args):

@@ -318,7 +318,8 @@ uint64_t plain_layout(void* base, uint32_t n, uint64_t nnz, bool host_in, bool w
     B.ooff = static_cast<uint64_t*>(cv.take((uint64_t(n) + 1) * 8));
     B.oheads = static_cast<uint32_t*>(cv.take((m + 4) * 4));  // +4: 16-byte block reads past the end
     B.ws_orient = static_cast<uint64_t*>(cv.take(orient_ws_words(n, nnz) * 8));
-    B.ws_tc = static_cast<uint64_t*>(cv.take(tc_ws_words(n) * 8));
+    // also the scratch of the longest-list reduction of unsorted input
+    B.ws_tc = static_cast<uint64_t*>(cv.take(std::max(tc_ws_words(n), segsort_tmp_words()) * 8));
     B.pv = want_pv_dev ? static_cast<uint64_t*>(cv.take(std::max<uint64_t>(n, 1) * 8)) : nullptr;
     B.keys = B.alt = B.ws_radix = nullptr;
     if (unsorted) {
@@ -416,7 +417,30 @@ int run_orientation(kt_ctx* c, uint32_t n, uint64_t nnz, const uint64_t* user_of
                        (unsigned long long)kept, (unsigned long long)nnz);
     }
     const uint64_t m = nnz / 2;
+    uint64_t max_len = 0;
     if (flags & KT_FLAG_UNSORTED) {
+        // lists of up to segsort_max_len() ids are sorted in place, per list;
+        // a longer one needs the global radix sort (and its buffers)
+        max_len_launch(n, B.ooff, B.ws_tc, B.small + 8, st);
+        rc = c->hip_check(hipMemcpyAsync(c->host_small + 8, B.small + 8, 8, hipMemcpyDeviceToHost, st),
+                          "D2H longest list");
+        if (rc) return rc;
+        rc = c->hip_check(hipStreamSynchronize(st), "longest list");
+        if (rc) return rc;
+        max_len = c->host_small[8];
+        trace("unsorted lists, longest oriented list %llu", (unsigned long long)max_len);
+    }
+    if ((flags & KT_FLAG_UNSORTED) && max_len <= segsort_max_len()) {
+        unsorted = true;
+        orient_write_launch(n, nnz, B.heads, B.oheads, B.ws_orient, st);
+        segsort_launch(n, B.ooff, B.oheads, reinterpret_cast<uint32_t*>(B.small), max_len, c->num_cus, st);
+        rc = c->hip_check(hipMemcpyAsync(c->host_small, B.small, 8, hipMemcpyDeviceToHost, st), "D2H flags");
+        if (rc) return rc;
+        rc = c->hip_check(hipStreamSynchronize(st), "sort");
+        if (rc) return rc;
+        flags = static_cast<uint32_t>(c->host_small[0]);
+        if (flags & KT_ERR_MASK) return c->fail(KT_EINVAL, "malformed CSR: duplicate edge");
+    } else if (flags & KT_FLAG_UNSORTED) {
         unsorted = true;
         need = plain_layout(nullptr, n, nnz, host_in, pv_dev_needed, true, B);
         if (need > ws.cap(c)) {

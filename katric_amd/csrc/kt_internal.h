@@ -51,6 +51,15 @@ void orient_write_launch(uint32_t n, uint64_t nnz, const uint32_t* heads, uint32
 void make_keys(uint32_t n, uint64_t m, const uint64_t* ooff, const uint32_t* oheads, uint64_t* keys,
                hipStream_t st);
 void keys_to_heads(uint64_t m, const uint64_t* keys, uint32_t* oheads, uint32_t* err, hipStream_t st);
+// Segmented in-place sort of the oriented lists oheads[ooff[v] .. ooff[v+1])
+// of up to segsort_max_len() ids each, max_len being the longest (duplicates
+// set KT_ERR_DUPLICATE); max_len_launch writes the longest list's length to
+// *out, using tmp (segsort_tmp_words() words).
+uint64_t segsort_tmp_words();
+void max_len_launch(uint32_t n, const uint64_t* ooff, uint64_t* tmp, uint64_t* out, hipStream_t st);
+uint32_t segsort_max_len();
+void segsort_launch(uint32_t n, const uint64_t* ooff, uint32_t* oheads, uint32_t* err, uint64_t max_len, int num_cus,
+                    hipStream_t st);
 
 // fill kernels (kt_util.hip); used instead of hipMemsetAsync so that every
 // pipeline stays capturable in a HIP graph.

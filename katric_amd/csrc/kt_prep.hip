@@ -339,16 +339,29 @@ __global__ void __launch_bounds__(OR_THREADS)
 
 // Folds the per-tile pair checksums (tiles entries) and the kept count into
 // the error flags: a symmetric CSR keeps exactly half of its slots and its
-// checksum is zero.  One 256-thread block.
-__global__ void __launch_bounds__(256) k_sym_final(const uint64_t* __restrict__ sym_part, uint64_t tiles,
-                                                   const uint64_t* __restrict__ kept, uint64_t nnz,
-                                                   uint32_t* __restrict__ err) {
-    __shared__ uint64_t lw[256 / KT_WAVE];
-    uint64_t s = 0;
-    for (uint64_t i = threadIdx.x; i < tiles; i += 256) s += sym_part[i];
-    uint64_t tot;
-    block_excl_sum(s, lw, &tot);
+// checksum is zero.  One 1024-thread block, eight independent loads in flight
+// per thread (a batch of 2^31 slots has half a million tiles).
+constexpr int SYM_FINAL_THREADS = 1024;
+__global__ void __launch_bounds__(SYM_FINAL_THREADS) k_sym_final(const uint64_t* __restrict__ sym_part,
+                                                                 uint64_t tiles, const uint64_t* __restrict__ kept,
+                                                                 uint64_t nnz, uint32_t* __restrict__ err) {
+    constexpr int U = 8;
+    __shared__ uint64_t lw[SYM_FINAL_THREADS / KT_WAVE];
+    uint64_t s[U] = {};
+    uint64_t i = threadIdx.x;
+    for (; i + (U - 1) * SYM_FINAL_THREADS < tiles; i += U * SYM_FINAL_THREADS)
+#pragma unroll
+        for (int k = 0; k < U; ++k) s[k] += sym_part[i + (uint64_t)k * SYM_FINAL_THREADS];
+    for (; i < tiles; i += SYM_FINAL_THREADS) s[0] += sym_part[i];
+#pragma unroll
+    for (int k = 1; k < U; ++k) s[0] += s[k];
+    const uint64_t w = wave_incl_sum(s[0]);
+    if (lane_id() == KT_WAVE - 1) lw[threadIdx.x / KT_WAVE] = w;
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    uint64_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < SYM_FINAL_THREADS / KT_WAVE; ++k) tot += lw[k];
     bool bad = tot != 0;
     if (kept) bad |= 2 * kept[0] != nnz;
     if (bad) atomicOr(err, KT_ERR_ASYMMETRIC);
@@ -464,6 +477,258 @@ __global__ void k_keys_to_heads(uint64_t m, const uint64_t* __restrict__ keys, u
     uint64_t k = keys[i];
     oheads[i] = (uint32_t)k;
     if (i + 1 < m && keys[i + 1] == k) atomicOr(err, KT_ERR_DUPLICATE);
+}
+
+// ---------------------------------------------------------------------------
+// segmented sort of the oriented lists (unsorted input)
+//
+// Every list N+(v) = oheads[ooff[v] .. ooff[v+1]) is sorted in place; lists
+// never mix, so no keys with row ids and no second buffer are needed.  By
+// length: up to 16 ids a thread sorts its own list in registers (a bitonic
+// network over 16 slots padded with the largest id); 17..1024 ids a wave
+// sorts the list in registers, K = 1..16 ids per lane (slot i = r * 64 +
+// lane: merge steps of distance >= 64 stay inside a lane, shorter ones are
+// lane shuffles); 1025..SEG_MAX ids a workgroup sorts it in LDS (bitonic over
+// the next power of two).  Longer lists go to the global radix sort.  Equal
+// neighbours after the sort are duplicate edges.
+// ---------------------------------------------------------------------------
+constexpr uint32_t SEG_THREAD_MAX = 16;
+constexpr uint32_t SEG_WAVE_MAX = 1024;
+constexpr uint32_t SEG_MAX = 4096;
+constexpr int SEG_BLOCK = 256;
+
+__device__ __forceinline__ void cas(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = min(a, b), hi = max(a, b);
+    a = lo;
+    b = hi;
+}
+
+// Sorts p[0 .. len) (len <= 64 K) with the whole wave; returns the duplicate
+// flag of this lane's slots.
+template <int K>
+__device__ __forceinline__ uint32_t wave_sort_list(uint32_t* __restrict__ p, uint32_t len, uint32_t lane) {
+    constexpr uint32_t P = 64 * K;
+    uint32_t x[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const uint32_t i = r * 64 + lane;
+        x[r] = i < len ? p[i] : 0xffffffffu;
+    }
+#pragma unroll
+    for (uint32_t k = 2; k <= P; k <<= 1)
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                // partner slot i ^ j is register r ^ (j / 64) of this lane;
+                // the direction bit (i & k) depends on r only
+#pragma unroll
+                for (int r = 0; r < K; ++r) {
+                    const int q = r ^ (int)(j / 64);
+                    if (q > r) {
+                        if ((r & (int)(k / 64)) == 0)
+                            cas(x[r], x[q]);
+                        else
+                            cas(x[q], x[r]);
+                    }
+                }
+            } else {
+                const bool lower = (lane & j) == 0;
+#pragma unroll
+                for (int r = 0; r < K; ++r) {
+                    const uint32_t y = (uint32_t)__shfl_xor((int)x[r], (int)j, KT_WAVE);
+                    const bool up = k < 64 ? (lane & k) == 0 : (r & (int)(k / 64)) == 0;
+                    x[r] = (lower == up) ? min(x[r], y) : max(x[r], y);
+                }
+            }
+        }
+    uint32_t f = 0;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const uint32_t i = r * 64 + lane;
+        if (i < len) p[i] = x[r];
+        // the next slot: the next lane's register r, or lane 0's register r + 1
+        uint32_t nx = (uint32_t)__shfl_down((int)x[r], 1, KT_WAVE);
+        if (r + 1 < K) {
+            const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)x[r + 1 < K ? r + 1 : r]);
+            if (lane == 63) nx = w;
+        }
+        if (i + 1 < len && nx == x[r]) f = KT_ERR_DUPLICATE;
+    }
+    return f;
+}
+
+// Rows of 64-row chunks, one chunk per wave: lists of <= 16 ids sorted by
+// their own lane, then the chunk's lists of 17..1024 ids one after another by
+// the whole wave.
+__global__ void __launch_bounds__(SEG_BLOCK) k_segsort_small(uint32_t n, const uint64_t* __restrict__ ooff,
+                                                             uint32_t* __restrict__ oheads,
+                                                             uint32_t* __restrict__ err) {
+    const uint32_t lane = lane_id();
+    const uint64_t nw = (uint64_t)gridDim.x * (SEG_BLOCK / KT_WAVE);
+    uint32_t flags = 0;
+    for (uint64_t c = (uint64_t)blockIdx.x * (SEG_BLOCK / KT_WAVE) + threadIdx.x / KT_WAVE; c * KT_WAVE < n; c += nw) {
+        const uint64_t v = c * KT_WAVE + lane;
+        uint64_t b = 0, e = 0;
+        if (v < n) {
+            b = ooff[v];
+            e = ooff[v + 1];
+        }
+        const uint32_t len = (uint32_t)(e - b);
+        if (len >= 2 && len <= SEG_THREAD_MAX) {
+            uint32_t a[SEG_THREAD_MAX];
+#pragma unroll
+            for (uint32_t j = 0; j < SEG_THREAD_MAX; ++j) a[j] = j < len ? oheads[b + j] : 0xffffffffu;
+            // bitonic network on 16 slots (static indices: stays in registers)
+#pragma unroll
+            for (uint32_t k = 2; k <= SEG_THREAD_MAX; k <<= 1)
+#pragma unroll
+                for (uint32_t j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                    for (uint32_t i = 0; i < SEG_THREAD_MAX; ++i) {
+                        const uint32_t l = i ^ j;
+                        if (l > i) {
+                            if ((i & k) == 0)
+                                cas(a[i], a[l]);
+                            else
+                                cas(a[l], a[i]);
+                        }
+                    }
+#pragma unroll
+            for (uint32_t j = 0; j < SEG_THREAD_MAX; ++j) {
+                if (j < len) oheads[b + j] = a[j];
+                if (j + 1 < len && a[j] == a[j + 1]) flags |= KT_ERR_DUPLICATE;
+            }
+        }
+        // the chunk's lists of 17..1024 ids, one at a time, by the whole wave
+        uint64_t mid = __ballot(len > SEG_THREAD_MAX && len <= SEG_WAVE_MAX);
+        while (mid) {
+            const int r = __ffsll((unsigned long long)mid) - 1;
+            mid &= mid - 1;
+            const uint64_t rb = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, r) |
+                                ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), r) << 32);
+            const uint32_t rl = (uint32_t)__builtin_amdgcn_readlane((int)len, r);
+            uint32_t* p = oheads + rb;
+            if (rl <= 64)
+                flags |= wave_sort_list<1>(p, rl, lane);
+            else if (rl <= 128)
+                flags |= wave_sort_list<2>(p, rl, lane);
+            else if (rl <= 256)
+                flags |= wave_sort_list<4>(p, rl, lane);
+            else if (rl <= 512)
+                flags |= wave_sort_list<8>(p, rl, lane);
+            else
+                flags |= wave_sort_list<16>(p, rl, lane);
+        }
+    }
+    or_flags(err, flags);
+}
+
+// Lists of 1025..SEG_MAX ids: 256-row chunks per block, each such list sorted
+// by the whole block in LDS (bitonic over the next power of two).
+__global__ void __launch_bounds__(SEG_BLOCK) k_segsort_large(uint32_t n, const uint64_t* __restrict__ ooff,
+                                                             uint32_t* __restrict__ oheads,
+                                                             uint32_t* __restrict__ err) {
+    __shared__ uint32_t s[SEG_MAX];
+    __shared__ uint32_t rows[SEG_BLOCK];
+    __shared__ uint32_t nrows;
+    __shared__ uint32_t bflags;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) bflags = 0;
+    for (uint64_t c0 = (uint64_t)blockIdx.x * SEG_BLOCK; c0 < n; c0 += (uint64_t)gridDim.x * SEG_BLOCK) {
+        if (tid == 0) nrows = 0;
+        __syncthreads();
+        const uint64_t v = c0 + tid;
+        if (v < n) {
+            const uint64_t len = ooff[v + 1] - ooff[v];
+            if (len > SEG_WAVE_MAX && len <= SEG_MAX) rows[atomicAdd(&nrows, 1u)] = (uint32_t)v;
+        }
+        __syncthreads();
+        const uint32_t cnt = nrows;
+        for (uint32_t r = 0; r < cnt; ++r) {
+            const uint32_t row = rows[r];
+            const uint64_t b = ooff[row];
+            const uint32_t len = (uint32_t)(ooff[row + 1] - b);
+            uint32_t P = 2 * SEG_WAVE_MAX;
+            while (P < len) P <<= 1;
+            for (uint32_t i = tid; i < P; i += SEG_BLOCK) s[i] = i < len ? oheads[b + i] : 0xffffffffu;
+            __syncthreads();
+            for (uint32_t k = 2; k <= P; k <<= 1)
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t i = tid; i < P; i += SEG_BLOCK) {
+                        const uint32_t l = i ^ j;
+                        if (l > i) {
+                            const uint32_t a = s[i], c = s[l];
+                            const bool up = (i & k) == 0;
+                            if ((a > c) == up) {
+                                s[i] = c;
+                                s[l] = a;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            uint32_t f = 0;
+            for (uint32_t i = tid; i < len; i += SEG_BLOCK) {
+                oheads[b + i] = s[i];
+                if (i + 1 < len && s[i] == s[i + 1]) f = KT_ERR_DUPLICATE;
+            }
+            if (f) atomicOr(&bflags, f);
+            __syncthreads();
+        }
+        __syncthreads();  // every thread has read nrows before it is reset
+    }
+    if (tid == 0 && bflags) or_flags(err, bflags);
+}
+
+// Largest oriented list (partials per block, then one block).
+__global__ void __launch_bounds__(256) k_max_len(uint32_t n, const uint64_t* __restrict__ ooff,
+                                                 uint32_t* __restrict__ part) {
+    uint32_t mx = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (uint64_t)gridDim.x * 256)
+    {
+        const uint64_t d = ooff[v + 1] - ooff[v];
+        mx = max(mx, d > 0xffffffffull ? 0xffffffffu : (uint32_t)d);
+    }
+    mx = wave_incl_max_u32(mx);
+    __shared__ uint32_t w[4];
+    if (lane_id() == KT_WAVE - 1) w[threadIdx.x / KT_WAVE] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = max(max(w[0], w[1]), max(w[2], w[3]));
+}
+
+__global__ void __launch_bounds__(256) k_max_final(uint32_t parts, const uint32_t* __restrict__ part,
+                                                   uint64_t* __restrict__ out) {
+    uint32_t mx = 0;
+    for (uint32_t i = threadIdx.x; i < parts; i += 256) mx = max(mx, part[i]);
+    mx = wave_incl_max_u32(mx);
+    __shared__ uint32_t w[4];
+    if (lane_id() == KT_WAVE - 1) w[threadIdx.x / KT_WAVE] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = max(max(w[0], w[1]), max(w[2], w[3]));
+}
+
+constexpr uint32_t MAXLEN_PARTS = 1024;
+
+uint64_t segsort_tmp_words() { return MAXLEN_PARTS / 2 + 1; }
+
+void max_len_launch(uint32_t n, const uint64_t* ooff, uint64_t* tmp, uint64_t* out, hipStream_t st) {
+    uint32_t* part = reinterpret_cast<uint32_t*>(tmp);
+    const uint32_t g = (uint32_t)std::min<uint64_t>(MAXLEN_PARTS, ((uint64_t)n + 255) / 256 + 1);
+    k_max_len<<<g, 256, 0, st>>>(n, ooff, part);
+    k_max_final<<<1, 256, 0, st>>>(g, part, out);
+}
+
+uint32_t segsort_max_len() { return SEG_MAX; }
+
+void segsort_launch(uint32_t n, const uint64_t* ooff, uint32_t* oheads, uint32_t* err, uint64_t max_len, int num_cus,
+                    hipStream_t st) {
+    if (n == 0 || max_len < 2) return;
+    const uint64_t chunks = ((uint64_t)n + KT_WAVE - 1) / KT_WAVE;
+    const uint32_t g1 = (uint32_t)std::min<uint64_t>((chunks + 3) / 4, (uint64_t)num_cus * 32);
+    k_segsort_small<<<g1, SEG_BLOCK, 0, st>>>(n, ooff, oheads, err);
+    if (max_len <= SEG_WAVE_MAX) return;
+    const uint32_t g2 = (uint32_t)std::min<uint64_t>(((uint64_t)n + SEG_BLOCK - 1) / SEG_BLOCK, (uint64_t)num_cus * 8);
+    k_segsort_large<<<g2, SEG_BLOCK, 0, st>>>(n, ooff, oheads, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -649,7 +914,7 @@ void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const ui
         fill_u64(w.tile_off, 1, 0, st);
     }
     scan_u64(w.dplus, ooff, n, w.stmp, st);
-    k_sym_final<<<1, 256, 0, st>>>(w.sym, w.tiles, ooff + n, nnz, err);
+    k_sym_final<<<1, SYM_FINAL_THREADS, 0, st>>>(w.sym, w.tiles, ooff + n, nnz, err);
 }
 
 uint64_t sym_ws_words(uint32_t n, uint64_t nnz) {
@@ -669,7 +934,7 @@ void sym_check_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint3
         k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, tbnd);
         k_sym_tiles<<<(unsigned)tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, tbnd, err, ck, sym);
     }
-    k_sym_final<<<1, 256, 0, st>>>(sym, (tiles > 0 && n > 0) ? tiles : 0, nullptr, nnz, err);
+    k_sym_final<<<1, SYM_FINAL_THREADS, 0, st>>>(sym, (tiles > 0 && n > 0) ? tiles : 0, nullptr, nnz, err);
 }
 
 void random_keys(uint64_t key[2]) {

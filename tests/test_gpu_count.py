@@ -90,6 +90,69 @@ def test_unsorted_input(ctx):
     np.testing.assert_array_equal(pv3, pv2)
 
 
+def _clique_minus_matching(n, dup=False):
+    """K_n minus the perfect matching (0,2), (1,3), (4,5), (6,7), ...: every
+    vertex has degree n - 2, so the (degree, id) orientation sends every edge
+    to the larger id and vertex 0's oriented list holds n - 2 ids.  With dup,
+    edges (0,4) and (1,5) are traded for (4,5) and a second copy of (0,1):
+    degrees stay n - 2 and vertex 0's oriented list (n - 2 ids) holds 1 twice."""
+    assert n % 2 == 0 and n >= 8
+    iu, ju = np.triu_indices(n, 1)
+    keep = ~((iu == 0) & (ju == 2)) & ~((iu == 1) & (ju == 3)) & ~((iu >= 4) & (iu % 2 == 0) & (ju == iu + 1))
+    e = np.stack([iu[keep], ju[keep]], 1)
+    if dup:
+        drop = ((e[:, 0] == 0) & (e[:, 1] == 4)) | ((e[:, 0] == 1) & (e[:, 1] == 5))
+        e = np.concatenate([e[~drop], [[4, 5], [0, 1]]])
+    src = np.concatenate([e[:, 0], e[:, 1]])
+    dst = np.concatenate([e[:, 1], e[:, 0]])
+    order = np.lexsort((dst, src))
+    off = np.zeros(n + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    return K.Graph(n, np.cumsum(off).astype(np.uint64), dst[order].astype(np.uint32))
+
+
+def _shuffle_rows(g, seed):
+    rng = np.random.default_rng(seed)
+    heads = g.heads.copy()
+    for v in range(g.n):
+        a, b = int(g.offsets[v]), int(g.offsets[v + 1])
+        heads[a:b] = rng.permutation(heads[a:b])
+    return K.Graph(g.n, g.offsets, heads)
+
+
+@pytest.mark.parametrize("n", [10, 18, 20, 68, 300, 4098, 4100])
+def test_unsorted_list_lengths(ctx, n):
+    # shuffled rows whose oriented lists run through every length from 0 to
+    # n - 2: lists of up to 4096 ids are sorted per list on the device (by a
+    # lane up to 16, a wave up to 64, a workgroup up to 4096), a longer one
+    # sends the whole graph to the radix sort (n = 4100)
+    g = _clique_minus_matching(n)
+    gs = _shuffle_rows(g, n)
+    og = ctx.orient(gs)
+    # equal degrees: every edge points to the larger id
+    rows = np.repeat(np.arange(n), np.diff(g.offsets.astype(np.int64)))
+    up = g.heads > rows
+    ooff = np.zeros(n + 1, np.uint64)
+    ooff[1:] = np.cumsum(np.bincount(rows[up], minlength=n))
+    np.testing.assert_array_equal(og.offsets, ooff)
+    np.testing.assert_array_equal(og.heads, g.heads[up])
+    # K_n minus a perfect matching: each of the n/2 missing edges was in n - 2
+    # triangles of K_n, and no triangle holds two of them
+    t, pv = ctx.count(gs)
+    assert t == n * (n - 1) * (n - 2) // 6 - (n // 2) * (n - 2)
+    assert int(pv.sum()) == 3 * t
+    if n <= 300:
+        ot, opv, _ = O.run_shmem(g)
+        assert t == ot
+        np.testing.assert_array_equal(pv, opv)
+    # a duplicate edge in vertex 0's list (n - 2 ids) is found by the sort
+    gd = _shuffle_rows(_clique_minus_matching(n, dup=True), n + 1)
+    with pytest.raises(K.KatricError) as e:
+        ctx.count(gd)
+    assert e.value.status == K.KT_EINVAL
+    assert ctx.count(gs)[0] == t
+
+
 def test_empty_and_tiny(ctx):
     assert ctx.count(K.Graph(0, np.zeros(1, np.uint64), np.zeros(0, np.uint32)))[0] == 0
     g = K.Graph(5, np.zeros(6, np.uint64), np.zeros(0, np.uint32))
