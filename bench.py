@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # Shard (RMAT-26 ef16 seed 1, 16 ranks, rank 0, CeTriC), the count phase's
 # intersection launches (k_tc_main + k_tc_big, local and contracted, binning
 # and k_tc_huge): FETCH_SIZE 268,105,215 KiB x 2.000 + WRITE_SIZE 12,663,654 KiB = 562.05 GB.
-MEASURED_TRAFFIC = {("plain", 22, 16, 1): 120.69e9, ("shard-cetric", 26, 16, 1, 16, 0): 562.05e9}
+MEASURED_TRAFFIC = {("plain", 22, 16, 1): 74.24e9, ("shard-cetric", 26, 16, 1, 16, 0): 562.05e9}
 
 
 
@@ -181,7 +181,12 @@ def run_shard(args):
     b_pre = 8 * (nl + 1) + 4 * len(s.heads) + 12 * ng + 12 * nmsg + 4 * mtot + 8 * (X + 1) + 4 * Mk
     b_count = 4 * W + 4 * (Mk + mtot) + 8 * (X + 1) + 8 * X
     c_ms = float(np.mean(cnt_ms))
-    achieved = b_count / (c_ms * 1e-3) / 1e9
+    # design bytes of the two intersection launches: every streamed T-list id
+    # (4 B), every S entry once (4 B per local edge and received id), one walk
+    # entry per local edge and received id (4 B id + 16 B T bounds), tail
+    # offsets and per-vertex counts (16 B per tail)
+    b_design = 4 * st["streamed_ids"] + 24 * (Mk + mtot) + 16 * X
+    achieved = b_design / (c_ms * 1e-3) / 1e9
     units = Mk + mtot
     result = {
         "metric": f"shard edges/s (exact per-rank triangle counts, {'DiTriC' if args.ditric else 'CeTriC'} "
@@ -209,6 +214,10 @@ def run_shard(args):
             "traffic": None if (args.params or args.ditric) else MEASURED_TRAFFIC.get(
                 ("shard-cetric", args.scale, args.edge_factor, args.seed, args.ranks, args.rank)),
             "kernel_ms": c_ms,
+            "bytes_per_launch": b_design,
+            "bytes": "4 * streamed_ids + 24 * (oriented_shard_edges + message_ids) + 16 * tails",
+            "algorithmic_effective_GBs": b_count / (c_ms * 1e-3) / 1e9,
+            "algorithmic_bytes": "4 * W + 4 * (oriented_shard_edges + message_ids) + 8 * (tails + 1) + 8 * tails",
         },
         "cpu_baseline": None,
     }
@@ -576,12 +585,15 @@ def run_batch(args):
     print(json.dumps(result), flush=True)
 
 
-def design_bytes(wedge_checks, m, n):
+def design_bytes(streamed, m, n, entries):
     """HBM bytes the hash-probe intersection must move per launch: every
-    streamed T(u) once per probe (4 B per id; the reference's wedge_checks),
-    every S(x) once when its table is built (4m), the tail offsets and the
-    per-vertex counts (16n)."""
-    return 4 * wedge_checks + 4 * m + 16 * n
+    streamed T-list id once (4 B; the library's streamed_ids, close to the
+    reference's wedge_checks when every oriented edge is intersected from its
+    first vertex and far smaller with min_side), every S(x) once when its
+    table is built (4m), per walk entry the neighbour id and the bounds of
+    its T list (4 + 16 B; entries = 2m for the min-side walk over whole input
+    rows, m otherwise), and the tail offsets and per-vertex counts (16n)."""
+    return 4 * streamed + 4 * m + 20 * entries + 16 * n
 
 
 def unsorted_leg(args, ctx, g, total):
@@ -664,6 +676,8 @@ def run_plain(args):
     stats = ctx.stats()
     W = stats["intersection_size_local"]
     WC = stats["wedge_checks"]
+    streamed = stats["streamed_ids"]
+    entries = nnz if ctx.params()["min_side"] else m
     kern_ms = []
     ctx.synchronize()
     t_start = time.perf_counter()
@@ -681,11 +695,11 @@ def run_plain(args):
     d_pv.free()
 
     # roofline of k_tc_main on the bytes this design must move (unit = one
-    # oriented edge: 4 B per id of T(u) streamed -- wedge_checks ids in all --
-    # plus 4 B of S(x) per kept slot, plus 16 B per vertex); the reference's
-    # intersection_size_local figure, which charges both lists per edge
-    # (B_count = 4W + 4m + 8(n+1) + 8n), is reported beside it
-    b_design = design_bytes(WC, m, n)
+    # oriented edge: 4 B per streamed T-list id -- streamed_ids in all -- plus
+    # 4 B of S(x) per kept slot, 20 B per walk entry and 16 B per vertex); the
+    # reference's intersection_size_local figure, which charges both lists
+    # per edge (B_count = 4W + 4m + 8(n+1) + 8n), is reported beside it
+    b_design = design_bytes(streamed, m, n, entries)
     b_count = 4 * W + 4 * m + 8 * (n + 1) + 8 * n
     b_pre = 16 * (n + 1) + 12 * m
     k_ms = float(np.mean(kern_ms))
@@ -709,7 +723,8 @@ def run_plain(args):
             "params": knobs,
             "workload": f"RMAT-{args.scale} ef{args.edge_factor} a=0.57 b=0.19 c=0.19 seed={args.seed}, "
                         f"symmetric CSR resident in HBM -> total + per-vertex uint64 in HBM",
-            "n": n, "m": m, "triangles": total, "W": W, "wedge_checks": WC, "csr_checksum": f"{checksum:016x}",
+            "n": n, "m": m, "triangles": total, "W": W, "wedge_checks": WC, "streamed_ids": streamed,
+            "walk_entries": entries, "csr_checksum": f"{checksum:016x}",
             "device_ms": tm["total_ms"], "prep_ms": tm["prep_ms"], "count_ms": tm["count_ms"],
             "pipeline_reference_bytes_GBs": pipeline_gbs,
         },
@@ -724,7 +739,7 @@ def run_plain(args):
             "traffic": None if args.params else MEASURED_TRAFFIC.get(("plain", args.scale, args.edge_factor, args.seed)),
             "kernel_ms": k_ms,
             "bytes_per_launch": b_design,
-            "bytes": "4 * wedge_checks + 4 * m + 16 * n",
+            "bytes": "4 * streamed_ids + 4 * m + 20 * walk_entries + 16 * n",
             "algorithmic_effective_GBs": b_count / (k_ms * 1e-3) / 1e9,
             "algorithmic_bytes": "4 * W + 4 * m + 8 * (n + 1) + 8 * n, W = intersection_size_local",
         },

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define KATRIC_ABI_VERSION 2
+#define KATRIC_ABI_VERSION 3
 
 /* status codes */
 #define KT_OK 0
@@ -63,6 +63,11 @@ typedef struct {
     uint64_t vertices;
     uint64_t input_slots;    /* entries of the symmetric input CSR */
     uint64_t skipped_nodes;  /* tails skipped with pseudo2core (cetric_edge_iterator.h:239-242) */
+    /* T-list ids the intersection streamed through its probes.  With
+     * kt_params.min_side = 0 this is wedge_checks less the lists of tails
+     * with a single out-neighbour (which cannot close a triangle and are not
+     * walked); min_side streams fewer.  No reference counterpart. */
+    uint64_t streamed_ids;
 } kt_stats;
 
 /* Device times of the last call, from HIP events on the context stream. */
@@ -103,6 +108,16 @@ typedef struct {
      * (kt_count, kt_list_triangles, kt_batch_count, serving): the shard
      * entry points return KT_EINVAL while it is set. */
     uint32_t pseudo2core;
+    /* 1 (default): kt_count, kt_batch_count and serving intersect every
+     * oriented edge (v, u) from the side whose out-list is shorter -- N+(u)
+     * probed against the table of N+(v) as in the reference, or N+(v)
+     * against the table of N+(u) -- so the streamed ids drop from
+     * sum |N+(u)| (wedge_checks) to sum min(|N+(v)|, |N+(u)|).  Counts,
+     * per-vertex counts and the reference's work counters are the same
+     * either way.  0: always from v (the reference's walk,
+     * cetric_edge_iterator.h:888-1066).  Listing and the shard entry points
+     * always walk from v.  No reference counterpart. */
+    uint32_t min_side;
 } kt_params;
 
 /* Create a context on HIP device `device`.  hbm_budget_bytes bounds the

@@ -262,13 +262,14 @@ class Context:
         return {f: getattr(p, f) for f, _ in KtParams._fields_}
 
     def set_params(self, wave_max_deg=None, group_max_deg=None, grid_per_cu=None, strategy=None, order=None,
-                   bsearch_min_deg=None, pseudo2core=None, **extra):
+                   bsearch_min_deg=None, pseudo2core=None, min_side=None, **extra):
         """Intersection knobs (see kt_params); unknown keys are rejected."""
         if extra:
             raise TypeError(f"unknown parameters: {sorted(extra)}")
         cur = self.params()
         new = dict(wave_max_deg=wave_max_deg, group_max_deg=group_max_deg, grid_per_cu=grid_per_cu,
-                   strategy=strategy, order=order, bsearch_min_deg=bsearch_min_deg, pseudo2core=pseudo2core)
+                   strategy=strategy, order=order, bsearch_min_deg=bsearch_min_deg, pseudo2core=pseudo2core,
+                   min_side=min_side)
         cur.update({k: int(v) for k, v in new.items() if v is not None})
         p = KtParams(**cur)
         self.check(lib().kt_set_params(self._h, C.byref(p)), "kt_set_params")

@@ -29,6 +29,7 @@ class KtStats(C.Structure):
         ("vertices", u64),
         ("input_slots", u64),
         ("skipped_nodes", u64),
+        ("streamed_ids", u64),
     ]
 
 
@@ -52,6 +53,7 @@ class KtParams(C.Structure):
         ("order", u32),
         ("bsearch_min_deg", u32),
         ("pseudo2core", u32),
+        ("min_side", u32),
     ]
 
 

@@ -243,7 +243,8 @@ extern "C" int kt_set_params(kt_ctx* c, const kt_params* p) {
     if (!c || !p) return KT_EINVAL;
     if (p->wave_max_deg < 2 || p->wave_max_deg > 256 || p->group_max_deg < p->wave_max_deg ||
         p->group_max_deg > 2048 || p->grid_per_cu < 1 || p->grid_per_cu > 32 || p->strategy > KT_STRATEGY_GROUPED2 ||
-        p->order > KT_ORDER_VERTEX || p->bsearch_min_deg < 1 || p->bsearch_min_deg > 8192 || p->pseudo2core > 1)
+        p->order > KT_ORDER_VERTEX || p->bsearch_min_deg < 1 || p->bsearch_min_deg > 8192 || p->pseudo2core > 1 ||
+        p->min_side > 1)
         return KT_EINVAL;
     std::lock_guard<std::mutex> lk(c->mu);
     c->prm.wave_max_deg = p->wave_max_deg;
@@ -253,6 +254,7 @@ extern "C" int kt_set_params(kt_ctx* c, const kt_params* p) {
     c->prm.order = p->order;
     c->prm.bsearch_min_deg = p->bsearch_min_deg;
     c->prm.pseudo2core = p->pseudo2core;
+    c->prm.min_side = p->min_side;
     return KT_OK;
 }
 
@@ -265,6 +267,7 @@ extern "C" int kt_get_params(const kt_ctx* c, kt_params* p) {
     p->order = c->prm.order;
     p->bsearch_min_deg = c->prm.bsearch_min_deg;
     p->pseudo2core = c->prm.pseudo2core;
+    p->min_side = c->prm.min_side;
     return KT_OK;
 }
 
@@ -527,6 +530,11 @@ int kt::count_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint3
     view.X = n;
     view.s_off = B.ooff;
     view.sheads = B.oheads;
+    if (c->prm.min_side) {
+        view.in_off = B.off;
+        view.in_heads = B.heads;
+        view.in_keep = orient_keep_bits(n, nnz, B.ws_orient);
+    }
     TcOut out{pv, totals};
     trace("intersection: X=%u pv=%p", n, (void*)pv);
     // one graph's wave tails vary in work: smaller grabs even out the end of
@@ -543,7 +551,7 @@ int kt::count_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint3
     uint64_t* tstats = tc_stats_ptr(B.ws_tc);
     rc = c->hip_check(hipMemcpyAsync(c->host_small + 1, totals, 8, hipMemcpyDeviceToHost, st), "D2H total");
     if (rc) return rc;
-    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tstats, 24, hipMemcpyDeviceToHost, st), "D2H stats");
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tstats, 32, hipMemcpyDeviceToHost, st), "D2H stats");
     if (rc) return rc;
     if (want_pv && !pv_dev) {
         rc = c->hip_check(hipMemcpyAsync(per_vertex_out, pv, uint64_t(n) * 8, hipMemcpyDeviceToHost, st),
@@ -559,6 +567,7 @@ int kt::count_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint3
     c->stats.wedge_checks = c->host_small[2];
     c->stats.intersection_size_local = c->host_small[3];
     c->stats.skipped_nodes = c->host_small[4];
+    c->stats.streamed_ids = c->host_small[5];
     c->timing.h2d_ms = c->elapsed(EV_START, EV_H2D);
     c->timing.prep_ms = c->elapsed(EV_H2D, EV_PREP);
     c->timing.count_ms = c->elapsed(EV_PREP, EV_COUNT);
@@ -619,7 +628,7 @@ extern "C" int kt_list_triangles(kt_ctx* c, uint32_t n, const uint64_t* offsets,
                       "intersection launch");
     if (rc) return rc;
     scan_u64(tcnt, tcnt, n, stmp, st);
-    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tc_stats_ptr(B.ws_tc), 24, hipMemcpyDeviceToHost, st),
+    rc = c->hip_check(hipMemcpyAsync(c->host_small + 2, tc_stats_ptr(B.ws_tc), 32, hipMemcpyDeviceToHost, st),
                       "D2H stats");
     if (rc) return rc;
     uint64_t total = 0;
@@ -630,6 +639,7 @@ extern "C" int kt_list_triangles(kt_ctx* c, uint32_t n, const uint64_t* offsets,
     c->stats.wedge_checks = c->host_small[2];
     c->stats.intersection_size_local = c->host_small[3];
     c->stats.skipped_nodes = c->host_small[4];
+    c->stats.streamed_ids = c->host_small[5];
     if (total > capacity)
         return c->fail(KT_ERANGE, "%llu triangles do not fit a buffer of %llu", (unsigned long long)total,
                        (unsigned long long)capacity);

@@ -137,6 +137,11 @@ hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, cons
     v.X = (uint32_t)N;
     v.s_off = B.ooff;
     v.sheads = B.oheads;
+    if (prm.min_side) {
+        v.in_off = B.off;
+        v.in_heads = B.heads;
+        v.in_keep = orient_keep_bits((uint32_t)N, nnz, B.ws_orient);
+    }
     TcOut o{pv, B.small + 2};
     const hipError_t le = tc_launch(v, o, prm, B.ws_tc, num_cus, st, nullptr, nullptr);
     if (le != hipSuccess) return le;
@@ -147,7 +152,7 @@ hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, cons
     (void)hipMemcpyAsync(h_totals, B.totals, (uint64_t)G * 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(h_small, B.small, 8, hipMemcpyDeviceToHost, st);
     (void)hipMemcpyAsync(h_small + 1, B.ooff + N, 8, hipMemcpyDeviceToHost, st);
-    (void)hipMemcpyAsync(h_small + 2, tc_stats_ptr(B.ws_tc), 24, hipMemcpyDeviceToHost, st);
+    (void)hipMemcpyAsync(h_small + 2, tc_stats_ptr(B.ws_tc), 32, hipMemcpyDeviceToHost, st);
     return hipGetLastError();
 }
 
@@ -299,6 +304,7 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
         c->stats.wedge_checks = c->host_small[2];
         c->stats.intersection_size_local = c->host_small[3];
         c->stats.skipped_nodes = c->host_small[4];
+        c->stats.streamed_ids = c->host_small[5];
         c->stats.oriented_edges = kept;
         c->stats.vertices = N;
         c->stats.input_slots = nnz;

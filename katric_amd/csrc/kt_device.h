@@ -42,22 +42,16 @@ __device__ __forceinline__ T wave_sum(T x) {
     return x;
 }
 
-// One-byte degree class, monotone in the degree: the degree itself below
-// 128, above it 4 classes per octave (the two bits after the leading one).
-// Equal classes below 128 mean equal degrees.
-__device__ __forceinline__ uint32_t deg_class(uint32_t d) {
-    if (d < 128) return d;
-    const uint32_t lg = 31 - __clz(d);  // >= 7
-    return 128 + 4 * (lg - 7) + ((d >> (lg - 2)) & 3);
+// Four-bit degree class, monotone in the degree: floor(log2 d) capped at 15
+// (0 for d < 2).  Two classes per byte, so the class table of n vertices is
+// n / 2 bytes and stays in the L2 beside the streamed heads; equal classes
+// are decided by the exact degrees.
+__device__ __forceinline__ uint32_t deg_class4(uint32_t d) {
+    return d < 2 ? 0u : min(15u, 31u - (uint32_t)__clz(d));
 }
 
-// (deg a, a) < (deg b, b) from one-byte degree classes, reading the exact
-// degrees only when the classes are equal above 127.
-__device__ __forceinline__ bool class_before(uint32_t ca, uint32_t a, uint32_t cb, uint32_t b, const uint32_t* deg) {
-    if (ca != cb) return ca < cb;
-    if (ca < 128) return a < b;
-    const uint32_t da = deg[a], db = deg[b];
-    return da < db || (da == db && a < b);
+__device__ __forceinline__ uint32_t class4_of(const uint8_t* __restrict__ ncls, uint32_t u) {
+    return (ncls[u >> 1] >> ((u & 1) * 4)) & 15u;
 }
 
 // 64-bit finaliser of MurmurHash3 (fmix64): a bijective mix in which every

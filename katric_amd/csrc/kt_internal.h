@@ -48,6 +48,9 @@ void sym_check_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint3
 void random_keys(uint64_t key[2]);
 void orient_write_launch(uint32_t n, uint64_t nnz, const uint32_t* heads, uint32_t* oheads, uint64_t* ws,
                          hipStream_t st);
+// The keep bitmask of the input slots (bit i: slot i is an out-edge) inside
+// the orientation workspace; valid after orient_count_launch.
+const uint64_t* orient_keep_bits(uint32_t n, uint64_t nnz, const uint64_t* ws);
 void make_keys(uint32_t n, uint64_t m, const uint64_t* ooff, const uint32_t* oheads, uint64_t* keys,
                hipStream_t st);
 void keys_to_heads(uint64_t m, const uint64_t* keys, uint32_t* oheads, uint32_t* err, hipStream_t st);
@@ -78,6 +81,7 @@ struct TcParams {
     uint32_t wave_grab = 32;  // wave-bin tails a wave takes per queue access (1..64), vertex order
     uint32_t bsearch_min_deg = 8192;  // |S(x)| above this: binary-search probes in global memory (<= 8192)
     uint32_t pseudo2core = 0; // skip tails with |S(x)| < 2 and count them (Config::pseudo2core)
+    uint32_t min_side = 1;    // plain counting: intersect every edge from the side with the shorter T list
 };
 
 // How a team walks the T lists of a chunk of neighbours: the whole wave on
@@ -124,6 +128,13 @@ struct TcView {
     // not in the contracted graph)
     const uint64_t* stat_off = nullptr;
     uint32_t stat_skip_empty = 0;
+    // min-side walks (plain counting views; nullable): the symmetric input
+    // CSR the orientation was built from and its keep bitmask (bit i: slot i
+    // is an out-edge).  in_hash_max is set by the launch.
+    const uint64_t* in_off = nullptr;
+    const uint32_t* in_heads = nullptr;
+    const uint64_t* in_keep = nullptr;
+    uint64_t in_hash_max = 0;
 };
 
 struct TcOut {

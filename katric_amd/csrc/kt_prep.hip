@@ -111,27 +111,31 @@ void scan_u64(const uint64_t* in, uint64_t* out, uint64_t len, uint64_t* tmp, hi
 // degrees + offset validation
 // ---------------------------------------------------------------------------
 __global__ void k_degree(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, uint32_t* __restrict__ deg,
-                         uint8_t* __restrict__ dcls, uint32_t* __restrict__ err) {
-    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                         uint8_t* __restrict__ ncls, uint32_t* __restrict__ err) {
+    // blockDim is a multiple of 64: vertices 2i and 2i+1 sit in one wave
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v == 0 && (off[0] != 0 || off[n] != nnz)) atomicOr(err, KT_ERR_OFFSETS);
-    if (v >= n) return;
-    uint64_t a = off[v], b = off[v + 1];
-    if (b < a || b - a > 0xffffffffull) {
-        atomicOr(err, KT_ERR_OFFSETS);
-        deg[v] = 0;
-        dcls[v] = 0;
-        return;
+    uint32_t c = 0;
+    if (v < n) {
+        const uint64_t a = off[v], b = off[v + 1];
+        uint32_t d = 0;
+        if (b < a || b - a > 0xffffffffull)
+            atomicOr(err, KT_ERR_OFFSETS);
+        else
+            d = (uint32_t)(b - a);
+        deg[v] = d;
+        c = deg_class4(d);
     }
-    deg[v] = (uint32_t)(b - a);
-    dcls[v] = (uint8_t)deg_class((uint32_t)(b - a));
+    const uint32_t c1 = (uint32_t)__shfl_down((int)c, 1, KT_WAVE);
+    if (v < n && (v & 1) == 0) ncls[v >> 1] = (uint8_t)(c | (v + 1 < n ? c1 << 4 : 0u));
 }
 
 // ---------------------------------------------------------------------------
-// orientation over 4096-slot tiles of the symmetric CSR
+// orientation over 2048-slot tiles of the symmetric CSR
 //
 // k_orient_keep: slots are visited coalesced (slot = tile0 + k*256 + tid).
 // The row of every slot is resolved in LDS: each row starting inside the tile
-// marks its first slot, and an inclusive max-scan over the 4096 marks gives
+// marks its first slot, and an inclusive max-scan over the 2048 marks gives
 // the row of every slot.  The keep predicate (deg v, v) < (deg u, u) is
 // evaluated once per slot and stored as a bitmask (one u64 word per 64
 // slots); per-row kept counts d+(v) and the per-tile kept count are written
@@ -139,7 +143,7 @@ __global__ void k_degree(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ 
 // only the bitmask and the tile bases.
 // ---------------------------------------------------------------------------
 constexpr int OR_THREADS = 256;
-constexpr int OR_ITEMS = 16;
+constexpr int OR_ITEMS = 8;
 constexpr int OR_TILE = OR_THREADS * OR_ITEMS;
 
 // (deg v, v) < (deg u, u): node_ordering::degree_outward with every vertex
@@ -211,7 +215,7 @@ struct CheckArgs {
 template <bool BATCH>
 __global__ void __launch_bounds__(OR_THREADS)
     k_orient_keep(uint32_t n, uint64_t nnz, const uint64_t* __restrict__ off, const uint32_t* __restrict__ heads,
-                  const uint32_t* __restrict__ deg, const uint8_t* __restrict__ dcls, const uint32_t* __restrict__ tbnd,
+                  const uint32_t* __restrict__ deg, const uint8_t* __restrict__ ncls, const uint32_t* __restrict__ tbnd,
                   uint64_t* __restrict__ keep_bits,
                   uint64_t* __restrict__ dplus, uint64_t* __restrict__ tile_cnt, uint32_t* __restrict__ err,
                   CheckArgs ck, uint64_t* __restrict__ sym_part) {
@@ -237,10 +241,11 @@ __global__ void __launch_bounds__(OR_THREADS)
     // come from the next lane (the last lane loads them), and kept slots are
     // counted per run of equal rows with one LDS atomic from the run's last lane.
     const unsigned lane = lane_id();
-    // all loads first (16 heads, then 32 one-byte degree-class gathers from
-    // an n-byte table that the L2 holds far better than the 4n-byte degrees),
-    // so that a thread has every item's memory traffic in flight at once
-    uint32_t hu[OR_ITEMS], cu[OR_ITEMS], cv[OR_ITEMS];
+    // all loads first (the heads, then one four-bit class gather per head
+    // from an n/2-byte table, half the L2 footprint of byte classes, and the
+    // rows' degrees), so that a thread has every item's memory traffic in
+    // flight at once
+    uint32_t hu[OR_ITEMS], cu[OR_ITEMS], dv[OR_ITEMS];
 #pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
         const uint64_t sl = tile0 + k * OR_THREADS + tid;
@@ -250,8 +255,8 @@ __global__ void __launch_bounds__(OR_THREADS)
     for (int k = 0; k < OR_ITEMS; ++k) {
         const unsigned i = k * OR_THREADS + tid;
         const bool in = tile0 + i < tend;
-        cu[k] = (in && hu[k] < n) ? dcls[hu[k]] : 0u;
-        cv[k] = in ? dcls[lo + marks[i]] : 0u;
+        cu[k] = (in && hu[k] < n) ? class4_of(ncls, hu[k]) : 0u;
+        dv[k] = in ? deg[lo + marks[i]] : 0u;
     }
     // the pair checksum while the class gathers are in flight: heads and
     // rows are at hand, and the VALU work hides behind the gathers' latency
@@ -260,6 +265,13 @@ __global__ void __launch_bounds__(OR_THREADS)
     for (int k = 0; k < OR_ITEMS; ++k) {
         const unsigned i = k * OR_THREADS + tid;
         if (tile0 + i < tend && hu[k] < n) s0 += pair_term(lo + marks[i], hu[k], ck.k0);
+    }
+    // equal classes: the head's exact degree (a second, rarer gather)
+    uint32_t du[OR_ITEMS];
+#pragma unroll
+    for (int k = 0; k < OR_ITEMS; ++k) {
+        const unsigned i = k * OR_THREADS + tid;
+        du[k] = (tile0 + i < tend && hu[k] < n && cu[k] == deg_class4(dv[k])) ? deg[hu[k]] : 0u;
     }
 #pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
@@ -275,9 +287,8 @@ __global__ void __launch_bounds__(OR_THREADS)
                 flags |= KT_ERR_HEAD_RANGE;
             } else {
                 if (u == v) flags |= KT_ERR_SELF_LOOP;
-                // classes decide unless they are equal above 127 (then the
-                // exact degrees, a rare gather)
-                keep = class_before(cv[k], v, cu[k], u, deg);
+                const uint32_t c = deg_class4(dv[k]);
+                keep = c != cu[k] ? c < cu[k] : keep_edge(v, dv[k], u, du[k]);
             }
         }
         // next slot in the same row? (shuffles outside any branch: every lane active)
@@ -859,7 +870,7 @@ struct OrientWs {
     uint64_t* dplus;     // n + 1
     uint32_t* tbnd;      // 2 * tiles: rows of each tile's first and last slot
     uint64_t* stmp;
-    uint8_t* dcls;       // n: degree classes
+    uint8_t* ncls;       // (n + 1) / 2: four-bit degree classes
 };
 
 static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
@@ -872,15 +883,19 @@ static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
     w.dplus = w.bits + (nnz + 63) / 64 + 1;
     w.tbnd = reinterpret_cast<uint32_t*>(w.dplus + (uint64_t)n + 1);
     w.stmp = w.dplus + (uint64_t)n + 1 + w.tiles;
-    w.dcls = reinterpret_cast<uint8_t*>(
+    w.ncls = reinterpret_cast<uint8_t*>(
         w.stmp + std::max(scan_tmp_words(w.tiles + 1), scan_tmp_words((uint64_t)n + 1)));
     return w;
+}
+
+const uint64_t* orient_keep_bits(uint32_t n, uint64_t nnz, const uint64_t* ws) {
+    return orient_ws(n, nnz, const_cast<uint64_t*>(ws)).bits;
 }
 
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz) {
     uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
     return (tiles + 1) + 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 + tiles +
-           std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1)) + ((uint64_t)n + 7) / 8;
+           std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1)) + ((uint64_t)n + 15) / 16;
 }
 
 // Rows of the first and last slot of every `tile`-slot tile of a CSR whose
@@ -897,17 +912,17 @@ void tile_bounds_launch(uint32_t n, uint64_t nnz, uint64_t tile, const uint64_t*
 void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint32_t* heads, uint32_t* deg,
                          uint64_t* ooff, uint64_t* ws, uint32_t* err, const OrientCheck& chk, hipStream_t st) {
     OrientWs w = orient_ws(n, nnz, ws);
-    k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, w.dcls, err);
+    k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, w.ncls, err);
     fill_u64(w.dplus, (uint64_t)n + 1, 0, st);
     if (w.tiles > 0) {
         const CheckArgs ck{chk.key[0], chk.key[1], chk.G ? chk.vbase : nullptr, chk.G};
         fill_u32(w.tbnd, 2 * w.tiles, 0, st);
         k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, w.tbnd);
         if (ck.vbase)
-            k_orient_keep<true><<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.dcls, w.tbnd,
+            k_orient_keep<true><<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.ncls, w.tbnd,
                                                                          w.bits, w.dplus, w.tile_cnt, err, ck, w.sym);
         else
-            k_orient_keep<false><<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.dcls, w.tbnd,
+            k_orient_keep<false><<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.ncls, w.tbnd,
                                                                           w.bits, w.dplus, w.tile_cnt, err, ck, w.sym);
         scan_u64(w.tile_cnt, w.tile_off, w.tiles, w.stmp, st);
     } else {

@@ -590,7 +590,7 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
         rc = c->hip_check(tc_launch(v, o, c->prm, B.tc_ws, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]),
                           "intersection launch");
         if (rc) return rc;
-        rc = c->hip_check(hipMemcpyAsync(c->host_small + 8, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
+        rc = c->hip_check(hipMemcpyAsync(c->host_small + 8, tstats, 32, hipMemcpyDeviceToHost, st), "D2H stats");
         if (rc) return rc;
     }
     // contracted / distributed phase
@@ -631,7 +631,7 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
         pm.team_chunk = std::min<uint32_t>(pm.team_chunk, cetric ? 4u : 2u);
         rc = c->hip_check(tc_launch(v, o, pm, B.tc_ws, c->num_cus, st, nullptr, nullptr), "intersection launch");
         if (rc) return rc;
-        rc = c->hip_check(hipMemcpyAsync(c->host_small + 12, tstats, 16, hipMemcpyDeviceToHost, st), "D2H stats");
+        rc = c->hip_check(hipMemcpyAsync(c->host_small + 12, tstats, 32, hipMemcpyDeviceToHost, st), "D2H stats");
         if (rc) return rc;
     }
     (void)hipEventRecord(c->ev[EV_COUNT], st);
@@ -651,6 +651,7 @@ int kt::count_shard_locked(kt_ctx* c, const kt_shard_in* in, kt_shard_out* out) 
     c->stats.wedge_checks = c->host_small[8] + (in->n_msg ? c->host_small[12] : 0);
     c->stats.intersection_size_local = c->host_small[9];
     c->stats.intersection_size_global = in->n_msg ? c->host_small[13] : 0;
+    c->stats.streamed_ids = c->host_small[11] + (in->n_msg ? c->host_small[15] : 0);
     c->stats.oriented_edges = Mk;
     c->stats.vertices = X;
     c->stats.input_slots = nnz;

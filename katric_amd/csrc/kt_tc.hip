@@ -62,8 +62,9 @@ constexpr uint32_t WAVE_GRAB = 8;  // wave-phase tails taken per queue access
 //   Q_PULL + 32 * b       pull cursor of bucket b
 //   Q_FILL + 32 * b       fill cursor of bucket b (binning pass 2)
 enum : int { Q_CNT = 0, Q_PULL = 32, Q_FILL = 32 * (NBUCKET + 1), Q_STRIDE = 32, Q_WORDS = 32 * (2 * NBUCKET + 2) };
-// stats words after the queue header (the reference's work counters)
-enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_SKIP = 2, ST_WORDS = 4 };
+// stats words after the queue header (the reference's work counters, and
+// the T-list ids the probes stream)
+enum : int { ST_WEDGE = 0, ST_ISZ = 1, ST_SKIP = 2, ST_STREAM = 3, ST_WORDS = 4 };
 
 template <bool PLAIN>
 struct View {
@@ -77,8 +78,18 @@ struct View {
             e = v.s_end[x];
         }
     }
+    // Min-side walk (plain views with in_off set): tail x walks its whole
+    // input row N(x) instead of N+(x).  The entry of an out-neighbour u
+    // streams T(u) here when |T(u)| <= |S(x)| (or when T(u)'s owner has no
+    // LDS table), the entry of an in-neighbour y streams T(y) here when
+    // |T(y)| < |S(x)| and S(x) is in an LDS table: every oriented edge is
+    // intersected exactly once, from the side whose streamed list is shorter.
+    __device__ __forceinline__ bool min_side() const { return PLAIN && v.in_off != nullptr; }
     __device__ __forceinline__ void r_range(uint32_t x, uint64_t& b, uint64_t& e) const {
-        if (PLAIN || v.r_beg == nullptr) {
+        if (min_side()) {
+            b = v.in_off[x];
+            e = v.in_off[x + 1];
+        } else if (PLAIN || v.r_beg == nullptr) {
             s_range(x, b, e);
         } else {
             b = v.r_beg[x];
@@ -98,10 +109,42 @@ struct View {
         }
     }
     __device__ __forceinline__ const uint32_t* t_heads() const { return PLAIN ? v.sheads : v.theads; }
-    // entry i of the walk: the neighbour u and the bounds of T(u)
-    __device__ __forceinline__ void walk_entry(uint64_t i, uint32_t& u, uint64_t& b, uint64_t& e) const {
-        u = r_head(i);
+    // Entry i of the walk, in two dependent steps so that the first can be
+    // issued a chunk ahead: entry_head loads the neighbour u (and, for a
+    // min-side walk, the keep-bitmask word of the slot); entry_bounds then
+    // gives the bounds of T(u) (empty when T(u) is not streamed by this tail,
+    // whose |S(x)| is ns) and what the entry adds to the reference's work
+    // counters (st_len to wedge_checks; st_cnt: one receiver for
+    // intersection_size).
+    __device__ __forceinline__ void entry_head(uint64_t i, uint32_t& u, uint32_t& kw) const {
+        if (min_side()) {
+            u = v.in_heads[i];
+            kw = reinterpret_cast<const uint32_t*>(v.in_keep)[i >> 5];
+        } else {
+            u = r_head(i);
+            kw = 0;
+        }
+    }
+    __device__ __forceinline__ void entry_bounds(uint64_t i, uint32_t u, uint32_t kw, uint64_t ns, uint64_t& b,
+                                                 uint64_t& e, uint64_t& st_len, bool& st_cnt) const {
         t_range(u, b, e);
+        if (min_side()) {
+            const bool fwd = (kw >> (i & 31)) & 1u;
+            const uint64_t dr = e - b;
+            st_len = fwd ? dr : 0;
+            st_cnt = fwd;
+            const bool here = fwd ? (dr <= ns || dr > v.in_hash_max) : (dr < ns && ns <= v.in_hash_max);
+            if (!here) e = b;
+            return;
+        }
+        st_len = e - b;
+        st_cnt = !(stat_skip_empty() && e == b);
+    }
+    __device__ __forceinline__ void walk_entry(uint64_t i, uint64_t ns, uint32_t& u, uint64_t& b, uint64_t& e,
+                                               uint64_t& st_len, bool& st_cnt) const {
+        uint32_t kw;
+        entry_head(i, u, kw);
+        entry_bounds(i, u, kw, ns, b, e, st_len, st_cnt);
     }
     __device__ __forceinline__ uint32_t tail_pv(uint32_t x) const {
         return (PLAIN || v.tail_id == nullptr) ? x : v.tail_id[x];
@@ -139,7 +182,10 @@ __device__ __forceinline__ int tail_bucket(const View<PLAIN>& V, const TcParams&
     if (s == 0 || r == 0 || ((PLAIN || prm.pseudo2core) && s < 2)) return -1;
     if (s > prm.bsearch_min_deg) return BUCKET_HUGE;
     if (s > prm.group_max_deg) return BUCKET_BIG;
-    const int b = (int)ceil_log2(s);  // 0..11
+    int b = (int)ceil_log2(s);  // 0..11
+    // a min-side walk covers the whole input row: a long row goes to a
+    // bigger bucket (a workgroup team beyond 512 entries), whatever |S(x)|
+    if (V.min_side()) b = max(b, min((int)ceil_log2(r) - 2, (int)ceil_log2(prm.group_max_deg)));
     if (prm.order == TC_BY_VERTEX) {  // two classes per XCD, each in vertex order
         const int xcd = (int)(((uint64_t)x * NXCD) / V.v.X);
         return b <= (int)ceil_log2(prm.wave_max_deg) ? NXCD + xcd : xcd;
@@ -168,9 +214,8 @@ __global__ void __launch_bounds__(256) k_tc_bin_count(View<PLAIN> V, TcParams pr
         // intersection in the reference's counters (cetric_edge_iterator.h:279-281)
         if (PLAIN && s == 1 && !prm.pseudo2core) {
             uint64_t sb, se, tb, te;
-            uint32_t u;
             V.s_range(x, sb, se);
-            V.walk_entry(sb, u, tb, te);
+            V.t_range(V.v.sheads[sb], tb, te);
             wedge += te - tb;
             isz += 1 + (te - tb);
         }
@@ -181,12 +226,13 @@ __global__ void __launch_bounds__(256) k_tc_bin_count(View<PLAIN> V, TcParams pr
             V.r_range(x, rb, re);
             const uint64_t len = V.stat_len(x, 0);
             for (uint64_t i = rb; i < re; ++i) {
-                uint64_t tb, te;
+                uint64_t tb, te, sl;
                 uint32_t u;
-                V.walk_entry(i, u, tb, te);
-                if (te == tb && V.stat_skip_empty()) continue;
-                wedge += te - tb;
-                isz += len + (te - tb);
+                bool sc;
+                V.walk_entry(i, 0, u, tb, te, sl, sc);
+                if (!sc) continue;
+                wedge += sl;
+                isz += len + sl;
             }
         }
     }
@@ -266,6 +312,7 @@ struct Lds {
     unsigned long long list_ctr[TC_WAVES];  // listing: triangles of the current tail written so far
     uint32_t next_u;
     uint32_t bcast;
+    uint32_t full;  // the team table has a full bucket
 };
 
 // Team control words of k_tc_big (its table is dynamic LDS).
@@ -275,6 +322,7 @@ struct LdsBig {
     unsigned long long list_ctr[BIG_WAVES];
     uint32_t next_u;
     uint32_t bcast;
+    uint32_t full;
 };
 
 template <int NW>
@@ -290,23 +338,38 @@ __device__ __forceinline__ void team_sync() {
 // hash(w) or, when that was full at insertion, in a later bucket (linear
 // probing over buckets).  A lookup therefore ends at the first bucket that
 // holds w or has an empty last slot.
-// Bucket of id w: w xor w rotated right by 12, masked -- three full-rate
-// instructions (a multiplicative hash costs a quarter-rate multiply).  The
+// Home bucket of id w as a byte offset into the table: w xor w rotated right
+// by 12, masked with boff_mask = (buckets - 1) << 4 -- two full-rate
+// instructions (one v_bitop3 after the rotation) that give the LDS address
+// offset directly (a multiplicative hash costs a quarter-rate multiply).  The
 // rotation folds the high bits onto the low ones, so ids that share their low
 // bits still spread; on vertex ids it fills buckets like a multiplicative hash.
-__device__ __forceinline__ uint32_t bucket_of(uint32_t w, uint32_t bmask) {
-    return (w ^ __builtin_amdgcn_alignbit(w, w, 12)) & bmask;
+__device__ __forceinline__ uint32_t bucket_off(uint32_t w, uint32_t boff_mask) {
+    return (w ^ __builtin_amdgcn_alignbit(w, w, 12)) & boff_mask;
 }
 
-__device__ __forceinline__ void table_insert(uint32_t* keys, uint32_t logB, uint32_t bmask, uint32_t w) {
-    uint32_t b = bucket_of(w, bmask);
+__device__ __forceinline__ uint32_t* at_byte(uint32_t* base, uint32_t off) {
+    return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(base) + off);
+}
+__device__ __forceinline__ const uint4* bucket_at(const uint32_t* keys, uint32_t off) {
+    return reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(keys) + off);
+}
+
+// Inserts w; returns true when w took the last slot of a bucket or went past
+// its home bucket: a table where no insertion did holds no full bucket, and
+// every lookup there ends in the home bucket.
+__device__ __forceinline__ bool table_insert(uint32_t* keys, uint32_t boff_mask, uint32_t w) {
+    uint32_t b = bucket_off(w, boff_mask);
+    bool moved = false;
     while (true) {
+        uint32_t* bk = at_byte(keys, b);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            uint32_t old = atomicCAS(&keys[4 * b + s], KT_EMPTY, w);
-            if (old == KT_EMPTY || old == w) return;
+            uint32_t old = atomicCAS(&bk[s], KT_EMPTY, w);
+            if (old == KT_EMPTY || old == w) return moved || s == 3;
         }
-        b = (b + 1) & bmask;
+        b = (b + 16) & boff_mask;
+        moved = true;
     }
 }
 
@@ -314,44 +377,48 @@ __device__ __forceinline__ void table_insert(uint32_t* keys, uint32_t logB, uint
 // lane's number of hits.  Each hit bumps the w-counter of its slot and calls
 // on_hit(e).  The four slot compares of a probe stay lane masks; hits are
 // counted per lane (one add), so the scalar unit only sees the branch masks.
+// Lookups continue past a full home bucket only when the table has one
+// (full_buckets, wave-uniform).
 template <class OnHit>
-__device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, uint32_t logB, uint32_t bmask,
-                                           const uint32_t (&w)[4], const bool (&valid)[4], OnHit on_hit) {
-    const uint4* kb = reinterpret_cast<const uint4*>(keys);
-    uint32_t bk[4];
+__device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, uint32_t boff_mask,
+                                           bool full_buckets, const uint32_t (&w)[4], const bool (&valid)[4],
+                                           OnHit on_hit) {
+    uint32_t bo[4];
     uint4 kk[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        bk[e] = bucket_of(w[e], bmask);
-        kk[e] = kb[bk[e]];
+        bo[e] = bucket_off(w[e], boff_mask);
+        kk[e] = *bucket_at(keys, bo[e]);
     }
     uint32_t nh = 0;
-    bool need[4];
+    bool hit[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const bool m0 = kk[e].x == w[e], m1 = kk[e].y == w[e], m2 = kk[e].z == w[e], m3 = kk[e].w == w[e];
-        const bool hit = valid[e] && (m0 || m1 || m2 || m3);
-        nh += hit ? 1u : 0u;
-        if (hit) {
-            atomicAdd(&cnt[4 * bk[e] + (m0 ? 0u : m1 ? 1u : m2 ? 2u : 3u)], 1u);
+        hit[e] = valid[e] && (m0 || m1 || m2 || m3);
+        nh += hit[e] ? 1u : 0u;
+        if (hit[e]) {
+            atomicAdd(at_byte(cnt, bo[e]) + (m0 ? 0u : m1 ? 1u : m2 ? 2u : 3u), 1u);
             on_hit(e);
         }
-        // a full home bucket without w: w may sit in a later bucket
-        need[e] = valid[e] && !hit && kk[e].w != KT_EMPTY;
     }
-    // rare at the table loads used: one branch for the four ids, skipped as a
-    // whole when no lane needs it
+    if (!full_buckets) return nh;
+    // a full home bucket without w: w may sit in a later bucket
+    bool need[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) need[e] = valid[e] && !hit[e] && kk[e].w != KT_EMPTY;
+    // one branch for the four ids, skipped as a whole when no lane needs it
     if (need[0] || need[1] || need[2] || need[3]) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             if (!need[e]) continue;
-            uint32_t b = bk[e];
+            uint32_t b = bo[e];
             while (true) {
-                b = (b + 1) & bmask;
-                const uint4 k = kb[b];
+                b = (b + 16) & boff_mask;
+                const uint4 k = *bucket_at(keys, b);
                 const bool n0 = k.x == w[e], n1 = k.y == w[e], n2 = k.z == w[e], n3 = k.w == w[e];
                 if (n0 || n1 || n2 || n3) {
-                    atomicAdd(&cnt[4 * b + (n0 ? 0u : n1 ? 1u : n2 ? 2u : 3u)], 1u);
+                    atomicAdd(at_byte(cnt, b) + (n0 ? 0u : n1 ? 1u : n2 ? 2u : 3u), 1u);
                     on_hit(e);
                     ++nh;
                     break;
@@ -424,12 +491,18 @@ struct Emitter {
     }
 };
 
+// Per-wave work counters: the reference's wedge_checks and
+// intersection_size, and the T-list ids this design streams through probes.
+struct WorkStats {
+    uint64_t wedge = 0, isz = 0, stream = 0;
+};
+
 struct ProbeCtx {
     Emitter em;  // listing launches only
     uint32_t* keys;
     uint32_t* cnt;
-    uint32_t logB;   // log2(buckets)
-    uint32_t bmask;  // buckets - 1
+    uint32_t boff;   // (buckets - 1) << 4: bucket byte-offset mask
+    bool full;       // some bucket is full (wave-uniform)
     const uint32_t* s;  // sorted S(x) (binary-search probes)
     uint64_t ns;
     uint64_t* pv;
@@ -444,20 +517,21 @@ struct ProbeCtx {
 // and list bounds are wave-uniform 32-bit values; hits are counted per lane
 // and summed across the wave once per list.  Returns the chunk's hits.
 template <bool HASH, bool PLAIN, bool LIST>
-__device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0, uint32_t cnt, const ProbeCtx& P,
-                                               uint64_t nstat, uint64_t& st_wedge, uint64_t& st_isz) {
+__device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0, uint32_t cnt, uint32_t uj,
+                                               uint32_t kw, const ProbeCtx& P, uint64_t nstat, WorkStats& wk) {
     const uint32_t lane = lane_id();
-    uint32_t uj = 0;
     uint64_t tbj = 0, tej = 0;
+    uint64_t stl = 0;
+    bool stc = false;
     if (lane < cnt) {
-        V.walk_entry(k0 + lane, uj, tbj, tej);
+        V.entry_bounds(k0 + lane, uj, kw, P.ns, tbj, tej, stl, stc);
     }
     uint64_t pending = __ballot(lane < cnt && tej > tbj);
     {
-        const uint64_t cw = uniform64(wave_sum(tej - tbj));
-        const uint32_t nrecv = V.stat_skip_empty() ? (uint32_t)__popcll(pending) : cnt;
-        st_wedge += cw;
-        st_isz += cw + (uint64_t)nrecv * nstat;
+        const uint64_t cw = uniform64(wave_sum(stl));
+        const uint32_t nrecv = (uint32_t)__popcll(__ballot(stc));
+        wk.wedge += cw;
+        wk.isz += cw + (uint64_t)nrecv * nstat;
     }
     if (pending == 0) return 0;
     const uint32_t* __restrict__ T = V.t_heads();
@@ -469,28 +543,32 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
     const uint32_t* base = T + readlane64(tbj, j);  // list start (uniform)
     // |T(u)| < 2^32: an out-list holds distinct 32-bit ids
     uint32_t len = (uint32_t)(readlane64(tej, j) - readlane64(tbj, j));
+    wk.stream += len;
     uint32_t pos = 0;  // block start within the list (uniform)
     // Every lane loads (lanes past the list end re-read its start; their ids
     // are masked by `valid`): an unconditional load keeps the number of
     // loads in flight static, so the prefetch below is not waited for early.
     // Addresses are a wave-uniform base plus a 32-bit lane offset.
     Ids4 q = load_ids4(base + (l4 < len ? l4 : 0u));
-    uint32_t nh = 0;  // this lane's hits on the current list
+    uint32_t nh = 0;    // this lane's hits on the current list
+    uint32_t uacc = 0;  // hits on this lane's own list (T(uj)), added at the list's end
     uint64_t total = 0;
     while (true) {
         // next block: same list, or the first block of the next non-empty list
         const uint32_t* nbase = base;
         uint32_t npos = pos + 4 * KT_WAVE, nlen = len, nu = u;
+        int nj = j;
         bool more = true, switch_u = false;
         if (npos >= len) {
             switch_u = true;
             if (pending) {
-                const int nj = __ffsll((unsigned long long)pending) - 1;
+                nj = __ffsll((unsigned long long)pending) - 1;
                 pending &= pending - 1;
                 nu = readlane32(uj, nj);
                 const uint64_t ntb = readlane64(tbj, nj);
                 nbase = T + ntb;
                 nlen = (uint32_t)(readlane64(tej, nj) - ntb);
+                wk.stream += nlen;
                 npos = 0;
             } else {
                 more = false;
@@ -503,9 +581,9 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
         if (HASH) {
             if constexpr (LIST)
-                nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, valid, [&](int e) { P.em(u, w[e]); });
+                nh += probe4(P.keys, P.cnt, P.boff, P.full, w, valid, [&](int e) { P.em(u, w[e]); });
             else
-                nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, valid, NoHitAction{});
+                nh += probe4(P.keys, P.cnt, P.boff, P.full, w, valid, NoHitAction{});
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -517,19 +595,20 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
         }
         if (switch_u) {
             const uint32_t h = wave_total_dpp(nh);
-            if (h) {
-                total += h;
-                if (lane == 0 && P.pv) atomic_add_u64(&P.pv[u], h);
-            }
+            total += h;
+            uacc += (lane == (uint32_t)j) ? h : 0u;
             nh = 0;
         }
         if (!more) break;
         u = nu;
+        j = nj;
         base = nbase;
         len = nlen;
         pos = npos;
         q = qn;
     }
+    // one atomic per neighbour with hits, issued once for the chunk
+    if (P.pv && uacc) atomic_add_u64(&P.pv[uj], uacc);
     return total;
 }
 
@@ -545,23 +624,24 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
 // active).  Hits are added to the neighbour's LDS counter as they occur and
 // flushed with one global atomic per neighbour.
 template <int G, bool PLAIN>
-__device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uint64_t k0, uint32_t cnt,
-                                                       const ProbeCtx& P, uint64_t nstat, uint64_t& st_wedge,
-                                                       uint64_t& st_isz, PackLds& pk) {
+__device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uint64_t k0, uint32_t cnt, uint32_t uj,
+                                                       uint32_t kw, const ProbeCtx& P, uint64_t nstat,
+                                                       WorkStats& wk, PackLds& pk) {
     constexpr uint32_t NQ = KT_WAVE / G;
     const uint32_t lane = lane_id();
-    uint32_t uj = 0;
-    uint64_t tbj = 0, tej = 0;
+    uint64_t tbj = 0, tej = 0, stl = 0;
+    bool stc = false;
     if (lane < cnt) {
-        V.walk_entry(k0 + lane, uj, tbj, tej);
+        V.entry_bounds(k0 + lane, uj, kw, P.ns, tbj, tej, stl, stc);
     }
     const bool nonempty = lane < cnt && tej > tbj;
     const uint64_t nem = __builtin_amdgcn_ballot_w64(nonempty);
     const uint32_t nne = (uint32_t)__popcll(nem);
     {
-        const uint64_t cw = uniform64(wave_sum(tej - tbj));
-        st_wedge += cw;
-        st_isz += cw + (uint64_t)(V.stat_skip_empty() ? nne : cnt) * nstat;
+        const uint64_t cw = uniform64(wave_sum(stl));
+        wk.wedge += cw;
+        wk.isz += cw + (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(stc)) * nstat;
+        wk.stream += uniform64(wave_sum(nonempty ? tej - tbj : 0ull));
     }
     if (nne == 0) return 0;
     // order[r] = lane of the r-th non-empty list
@@ -606,7 +686,7 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
         const int lim = (int)clen - (int)(pos + s4);
         const bool valid[4] = {0 < lim, 1 < lim, 2 < lim, 3 < lim};
         const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
-        nh += probe4(P.keys, P.cnt, P.logB, P.bmask, w, valid, [&](int) { atomicAdd(&pk.ucnt[csrc], 1u); });
+        nh += probe4(P.keys, P.cnt, P.boff, P.full, w, valid, [&](int) { atomicAdd(&pk.ucnt[csrc], 1u); });
         // advance; a group that switched lists fetches its list after next
         uint32_t fsrc, flen;
         const uint32_t* fbase;
@@ -649,7 +729,7 @@ template <int NW, bool HASH, bool PLAIN, int GL, bool LIST, class LdsT>
 __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, const TailBounds& tb, LdsT& L,
                                              uint32_t* keys, uint32_t* cnt, uint32_t cap_slots, uint32_t shift,
                                              uint32_t chunk, const TcOut& out,
-                                             uint64_t& my_total, uint64_t& st_wedge, uint64_t& st_isz) {
+                                             uint64_t& my_total, WorkStats& wk) {
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / KT_WAVE;
     const unsigned twave = NW > 1 ? wave : 0;
@@ -663,8 +743,8 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, c
     ProbeCtx P;
     P.keys = keys;
     P.cnt = cnt;
-    P.logB = 0;
-    P.bmask = 0;
+    P.boff = 0;
+    P.full = false;
     P.s = V.v.sheads + sb;
     P.ns = ns;
     P.pv = out.per_vertex;
@@ -677,14 +757,23 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, c
             slots = cap_slots;
             logS = ceil_log2(cap_slots);
         }
-        P.logB = logS - 2;
-        P.bmask = (slots >> 2) - 1;
+        P.boff = ((slots >> 2) - 1) << 4;
         for (uint32_t i = tid; i < slots; i += NW * KT_WAVE) {
             keys[i] = KT_EMPTY;
             cnt[i] = 0;
         }
         team_sync<NW>();
-        for (uint64_t i = tid; i < ns; i += NW * KT_WAVE) table_insert(keys, P.logB, P.bmask, V.v.sheads[sb + i]);
+        bool full = false;
+        for (uint64_t i = tid; i < ns; i += NW * KT_WAVE) full |= table_insert(keys, P.boff, V.v.sheads[sb + i]);
+        if (NW > 1) {
+            if (tid == 0) L.full = 0;
+            team_sync<NW>();
+            if (__builtin_amdgcn_ballot_w64(full) && lane == 0) atomicOr(&L.full, 1u);
+            team_sync<NW>();
+            P.full = uniform(L.full) != 0;
+        } else {
+            P.full = __builtin_amdgcn_ballot_w64(full) != 0;
+        }
     }
     if (NW > 1 && tid == 0) L.next_u = NW * chunk;
     if (LIST) {
@@ -695,19 +784,27 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, c
 
     uint64_t tail_hits = 0;
     uint64_t k = (uint64_t)twave * chunk;
+    // the neighbours of the next chunk are loaded while the current one is
+    // scanned (the bounds of their T lists depend on them)
+    uint32_t u_nx = 0, kw_nx = 0;
+    if (k < nr && lane < min<uint64_t>(chunk, nr - k)) V.entry_head(rb + k + lane, u_nx, kw_nx);
     while (k < nr) {
         const uint32_t c = (uint32_t)min<uint64_t>(chunk, nr - k);
-        if (GL < KT_WAVE && HASH)
-            tail_hits += scan_chunk_grouped<GL, PLAIN>(V, rb + k, c, P, nstat, st_wedge, st_isz, L.pk[wave]);
-        else
-            tail_hits += scan_chunk<HASH, PLAIN, LIST>(V, rb + k, c, P, nstat, st_wedge, st_isz);
+        const uint32_t u_cur = u_nx, kw_cur = kw_nx;
+        uint64_t nk;
         if (NW > 1) {
-            uint32_t nk = 0;
-            if (lane == 0) nk = atomicAdd(&L.next_u, chunk);
-            k = uniform(nk);
+            uint32_t g = 0;
+            if (lane == 0) g = atomicAdd(&L.next_u, chunk);
+            nk = uniform(g);
         } else {
-            k += chunk;
+            nk = k + chunk;
         }
+        if (nk < nr && lane < min<uint64_t>(chunk, nr - nk)) V.entry_head(rb + nk + lane, u_nx, kw_nx);
+        if (GL < KT_WAVE && HASH)
+            tail_hits += scan_chunk_grouped<GL, PLAIN>(V, rb + k, c, u_cur, kw_cur, P, nstat, wk, L.pk[wave]);
+        else
+            tail_hits += scan_chunk<HASH, PLAIN, LIST>(V, rb + k, c, u_cur, kw_cur, P, nstat, wk);
+        k = nk;
     }
 
     // team reduction of the tail's hits
@@ -750,7 +847,8 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
     __shared__ Lds L;
     const unsigned wave = threadIdx.x / KT_WAVE;
     const unsigned lane = lane_id();
-    uint64_t my_total = 0, st_wedge = 0, st_isz = 0;
+    uint64_t my_total = 0;
+    WorkStats wk;
     const int wave_bucket_max = (int)ceil_log2(prm.wave_max_deg);
 
     const bool by_vertex = prm.order == TC_BY_VERTEX;
@@ -776,8 +874,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             if (i >= cnt) break;
             const uint32_t x = list[base + i];
             team_process<TC_WAVES, true, PLAIN, GL, LIST>(V, x, tail_bounds(V, x), L, L.keys, L.cnt, GROUP_SLOTS,
-                                                          prm.table_shift, prm.team_chunk, out, my_total, st_wedge,
-                                                          st_isz);
+                                                          prm.table_shift, prm.team_chunk, out, my_total, wk);
         }
     }
     // wave phase: independent waves with private tables; by degree, largest
@@ -818,7 +915,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
                     const TailBounds tb{readlane64(bl.sb, j), readlane64(bl.se, j), readlane64(bl.rb, j),
                                         readlane64(bl.re, j), readlane64(bl.nstat, j)};
                     team_process<1, true, PLAIN, GL, LIST>(V, readlane32(xl, j), tb, L, keys, tcnt, WAVE_SLOTS,
-                                                           prm.table_shift, KT_WAVE, out, my_total, st_wedge, st_isz);
+                                                           prm.table_shift, KT_WAVE, out, my_total, wk);
                 }
             }
         }
@@ -827,22 +924,26 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
     __syncthreads();
     if (lane == 0) {
         L.red[wave] = my_total;
-        L.keys[wave] = (uint32_t)st_wedge;
-        L.keys[TC_WAVES + wave] = (uint32_t)(st_wedge >> 32);
-        L.cnt[wave] = (uint32_t)st_isz;
-        L.cnt[TC_WAVES + wave] = (uint32_t)(st_isz >> 32);
+        L.keys[wave] = (uint32_t)wk.wedge;
+        L.keys[TC_WAVES + wave] = (uint32_t)(wk.wedge >> 32);
+        L.cnt[wave] = (uint32_t)wk.isz;
+        L.cnt[TC_WAVES + wave] = (uint32_t)(wk.isz >> 32);
+        L.keys[2 * TC_WAVES + wave] = (uint32_t)wk.stream;
+        L.keys[3 * TC_WAVES + wave] = (uint32_t)(wk.stream >> 32);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t t = 0, w = 0, z = 0;
+        uint64_t t = 0, w = 0, z = 0, y = 0;
         for (int i = 0; i < TC_WAVES; ++i) {
             t += L.red[i];
             w += ((uint64_t)L.keys[TC_WAVES + i] << 32) | L.keys[i];
             z += ((uint64_t)L.cnt[TC_WAVES + i] << 32) | L.cnt[i];
+            y += ((uint64_t)L.keys[3 * TC_WAVES + i] << 32) | L.keys[2 * TC_WAVES + i];
         }
         if (t) atomic_add_u64(&out.totals[0], t);
         if (w) atomic_add_u64(&stats[ST_WEDGE], w);
         if (z) atomic_add_u64(&stats[ST_ISZ], z);
+        if (y) atomic_add_u64(&stats[ST_STREAM], y);
     }
 }
 
@@ -854,7 +955,8 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
                                                          const uint32_t* __restrict__ list,
                                                          uint64_t* __restrict__ stats) {
     __shared__ Lds L;
-    uint64_t my_total = 0, st_wedge = 0, st_isz = 0;
+    uint64_t my_total = 0;
+    WorkStats wk;
     const uint32_t cnt = q[Q_CNT + BUCKET_HUGE];
     const uint32_t base = bucket_base(q, BUCKET_HUGE);
     uint32_t* pull = &q[Q_PULL + Q_STRIDE * BUCKET_HUGE];
@@ -869,12 +971,13 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
         if (i >= cnt) break;
         const uint32_t x = list[base + i];
         team_process<TC_WAVES, false, PLAIN, KT_WAVE, LIST>(V, x, tail_bounds(V, x), L, nullptr, nullptr, 0, 0,
-                                                            prm.team_chunk, out, my_total, st_wedge, st_isz);
+                                                            prm.team_chunk, out, my_total, wk);
     }
     if (lane_id() == 0) {
         if (my_total) atomic_add_u64(&out.totals[0], my_total);
-        if (st_wedge) atomic_add_u64(&stats[ST_WEDGE], st_wedge);
-        if (st_isz) atomic_add_u64(&stats[ST_ISZ], st_isz);
+        if (wk.wedge) atomic_add_u64(&stats[ST_WEDGE], wk.wedge);
+        if (wk.isz) atomic_add_u64(&stats[ST_ISZ], wk.isz);
+        if (wk.stream) atomic_add_u64(&stats[ST_STREAM], wk.stream);
     }
 }
 
@@ -887,7 +990,8 @@ __global__ void __launch_bounds__(BIG_THREADS) k_tc_big(View<PLAIN> V, TcOut out
                                                         uint64_t* __restrict__ stats) {
     __shared__ LdsBig L;
     extern __shared__ uint32_t big_table[];  // keys[BIG_SLOTS], then cnt[BIG_SLOTS]
-    uint64_t my_total = 0, st_wedge = 0, st_isz = 0;
+    uint64_t my_total = 0;
+    WorkStats wk;
     const uint32_t cnt = q[Q_CNT + BUCKET_BIG];
     const uint32_t base = bucket_base(q, BUCKET_BIG);
     uint32_t* pull = &q[Q_PULL + Q_STRIDE * BUCKET_BIG];
@@ -903,12 +1007,13 @@ __global__ void __launch_bounds__(BIG_THREADS) k_tc_big(View<PLAIN> V, TcOut out
         const uint32_t x = list[base + i];
         team_process<BIG_WAVES, true, PLAIN, KT_WAVE, LIST>(V, x, tail_bounds(V, x), L, big_table,
                                                             big_table + BIG_SLOTS, BIG_SLOTS, prm.table_shift,
-                                                            prm.team_chunk, out, my_total, st_wedge, st_isz);
+                                                            prm.team_chunk, out, my_total, wk);
     }
     if (lane_id() == 0) {
         if (my_total) atomic_add_u64(&out.totals[0], my_total);
-        if (st_wedge) atomic_add_u64(&stats[ST_WEDGE], st_wedge);
-        if (st_isz) atomic_add_u64(&stats[ST_ISZ], st_isz);
+        if (wk.wedge) atomic_add_u64(&stats[ST_WEDGE], wk.wedge);
+        if (wk.isz) atomic_add_u64(&stats[ST_ISZ], wk.isz);
+        if (wk.stream) atomic_add_u64(&stats[ST_STREAM], wk.stream);
     }
 }
 
@@ -928,6 +1033,9 @@ static hipError_t launch_impl(const TcView& view, const TcOut& out, const TcPara
     (void)hipGetLastError();  // a status left by an earlier call is not this launch's
     View<PLAIN> V;
     V.v = view;
+    // min-side walks move triangles between tails: listing launches (and
+    // their per-tail counting pass) keep every triangle at its first vertex
+    if (out.tri || out.tail_count) V.v.in_off = nullptr;
     uint32_t* q = reinterpret_cast<uint32_t*>(ws);
     uint64_t* stats = ws + Q_WORDS / 2;
     uint32_t* list = reinterpret_cast<uint32_t*>(stats + ST_WORDS);
@@ -940,6 +1048,11 @@ static hipError_t launch_impl(const TcView& view, const TcOut& out, const TcPara
     uint32_t wm = 1;
     while (wm < p.wave_max_deg) wm <<= 1;
     p.wave_max_deg = wm;
+    // |S| up to this is probed in an LDS table (k_tc_main or k_tc_big)
+    V.v.in_hash_max = p.bsearch_min_deg;
+    // a min-side walk covers in- and out-neighbours and streams from only
+    // some of them: four times the entries per team grab
+    if (V.v.in_off) p.team_chunk = std::min<uint32_t>(4 * std::max<uint32_t>(p.team_chunk, 1), KT_WAVE);
     if (view.X > 0) {
         const unsigned g = std::min<unsigned>((view.X + 255) / 256, (unsigned)num_cus * 8);
         k_tc_bin_count<PLAIN><<<g, 256, 0, st>>>(V, p, q, stats);

@@ -114,3 +114,63 @@ def test_set_params_rejects_bad_knobs(ctx):
         ctx.set_params(strategy=7)
     with pytest.raises(K.KatricError):
         ctx.set_params(order=2)
+    with pytest.raises(K.KatricError):
+        ctx.set_params(min_side=2)
+
+
+def _expected_streamed(g, min_side=True, hash_max=8192):
+    """Streamed T-list ids, from the oracle's oriented CSR.  Forward walk:
+    every oriented edge (x, u) streams |N+(u)| from x, except where x has a
+    single out-neighbour (no triangle can close there; the reference still
+    charges those lists to wedge_checks).  Min-side walk: |N+(u)| from x
+    when |N+(u)| <= |N+(x)| or u's set is beyond the LDS tables, else
+    |N+(x)| from u."""
+    ooff, oh = O.orient(g)
+    d = np.diff(ooff.astype(np.int64))
+    x = np.repeat(np.arange(g.n), d)
+    dx, du = d[x], d[oh]
+    fwd = np.where(dx >= 2, du, 0)
+    if not min_side:
+        return int(fwd.sum())
+    at_x = (du <= dx) | (du > hash_max)
+    return int(np.where(at_x, fwd, dx).sum())
+
+
+@pytest.mark.parametrize("bins", BINS)
+def test_min_side_walk(ctx, knobs, bins):
+    # min_side moves every edge's intersection to the side with the shorter
+    # T list: totals, per-vertex counts and the reference's work counters
+    # stay those of the forward walk (and the oracle's); only streamed_ids
+    # changes, to the closed form above
+    for strategy in (K.STRATEGY_STREAMED, K.STRATEGY_GROUPED8):
+        knobs(strategy, *bins)
+        for name, g in GRAPHS:
+            ctx.set_params(min_side=0)
+            t0, pv0 = ctx.count(g)
+            s0 = ctx.stats()
+            ctx.set_params(min_side=1)
+            t1, pv1 = ctx.count(g)
+            s1 = ctx.stats()
+            ot, opv, ost = O.run_shmem(g, threads=4)
+            assert t0 == t1 == ot, name
+            np.testing.assert_array_equal(pv0, opv)
+            np.testing.assert_array_equal(pv1, opv)
+            for k in ("wedge_checks", "intersection_size_local"):
+                assert s0[k] == s1[k] == ost[k], (name, k)
+            assert s0["streamed_ids"] == _expected_streamed(g, min_side=False), name
+            assert s1["streamed_ids"] == _expected_streamed(g), name
+
+
+def test_min_side_batch(ctx):
+    b = KB.gen_batch(48, seed=9, n_min=64, n_max=4000)
+    ctx.set_params(min_side=0)
+    t0, pv0 = KB.count_batch(ctx, b, per_vertex=True)
+    ctx.set_params(min_side=1)
+    t1, pv1 = KB.count_batch(ctx, b, per_vertex=True)
+    np.testing.assert_array_equal(t0, t1)
+    np.testing.assert_array_equal(pv0, pv1)
+    for g in range(0, b.num_graphs, 7):
+        t, opv, _ = O.run_shmem(b.graph(g))
+        assert t1[g] == t
+        a, z = int(b.vertex_base[g]), int(b.vertex_base[g + 1])
+        np.testing.assert_array_equal(pv1[a:z], opv)
