@@ -1,7 +1,8 @@
 """Autotuner for the intersection knobs (kt_params).
 
 The knobs -- wave/workgroup bin thresholds, how a team walks its T lists
-(streamed or grouped), queue order (degree or vertex), persistent workgroups per CU -- move work between
+(streamed or grouped), from which side each edge is intersected (min_side),
+queue order (degree or vertex), persistent workgroups per CU -- move work between
 kernels and code paths and never change a count.  Their best setting depends
 on the degree profile of the input, so the tuner
 
@@ -30,7 +31,7 @@ from typing import Callable, Dict, Iterable, List, Optional
 import numpy as np
 
 PACKAGE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_params.json")
-DEFAULTS = {"wave_max_deg": 128, "group_max_deg": 2048, "grid_per_cu": 4, "strategy": 0, "order": 0}
+DEFAULTS = {"wave_max_deg": 128, "group_max_deg": 2048, "grid_per_cu": 4, "strategy": 0, "order": 0, "min_side": 1}
 
 
 def user_table_path() -> str:
@@ -59,8 +60,9 @@ def batch_key(num_graphs: int, offsets: np.ndarray) -> str:
 
 
 STAGES = [
-    # stage 1: how lists are walked and in which order tails are taken
-    {"strategy": [0, 1, 2, 3, 4, 5], "order": [0, 1]},
+    # stage 1: how lists are walked, from which side of each edge, and in
+    # which order tails are taken
+    {"strategy": [0, 1, 2, 3, 4, 5], "order": [0, 1], "min_side": [1, 0]},
     # stage 2: the bin thresholds and the grid, under the best stage-1 choice
     {"wave_max_deg": [32, 64, 128, 256], "group_max_deg": [1024, 2048], "grid_per_cu": [2, 3, 4]},
 ]

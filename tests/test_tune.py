@@ -17,7 +17,7 @@ def test_candidates_respect_bin_order():
         for p in T.candidates(stage, T.DEFAULTS):
             assert p["group_max_deg"] >= p["wave_max_deg"]
             assert set(p) == set(T.DEFAULTS)
-    assert len(T.candidates(T.STAGES[0], T.DEFAULTS)) == 12  # 6 list walks x 2 orders
+    assert len(T.candidates(T.STAGES[0], T.DEFAULTS)) == 24  # 6 list walks x 2 orders x 2 walk sides
 
 
 def test_save_lookup_nearest(tmp_path):
