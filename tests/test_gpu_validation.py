@@ -63,6 +63,35 @@ def test_counterexample_kept_count_balanced():
     assert 2 * kept_slots(COUNTER) == COUNTER.nnz
 
 
+def test_symmetry_exhaustive_4_vertices(ctx):
+    # every CSR on 4 vertices whose rows are sorted subsets of the other
+    # vertices (8^4 = 4096 inputs): kt_count accepts exactly the symmetric
+    # ones -- out-lists and in-lists agree, the reference's orientation
+    # contract (tests/orientation_test.cpp:81-133) -- and counts them right
+    import itertools
+    n = 4
+    others = [[u for u in range(n) if u != v] for v in range(n)]
+    rows_of = [[[o[i] for i in range(3) if mask >> i & 1] for mask in range(8)] for o in others]
+    accepted = 0
+    for pick in itertools.product(range(8), repeat=n):
+        rows = [rows_of[v][pick[v]] for v in range(n)]
+        off = np.zeros(n + 1, np.uint64)
+        off[1:] = np.cumsum([len(r) for r in rows])
+        heads = np.array([u for r in rows for u in r], np.uint32)
+        g = K.Graph(n, off, heads)
+        sym = all((v in rows[u]) for v in range(n) for u in rows[v])
+        if sym:
+            t = ctx.count(g, per_vertex=False)[0]
+            assert t == sum(1 for a, b, c in itertools.combinations(range(n), 3)
+                            if b in rows[a] and c in rows[a] and c in rows[b]), rows
+            accepted += 1
+        else:
+            with pytest.raises(K.KatricError) as e:
+                ctx.count(g, per_vertex=False)
+            assert e.value.status == K.KT_EINVAL, rows
+    assert accepted == 2 ** 6  # one symmetric CSR per edge subset of K_4
+
+
 @pytest.mark.parametrize("case", ["n4", "rmat12-planted"])
 def test_asymmetric_plain_paths(ctx, case):
     g = COUNTER if case == "n4" else planted_asymmetric()
