@@ -10,7 +10,9 @@
 // The pipeline has no host synchronisation between its kernels: the oriented
 // heads buffer holds nnz entries (an asymmetric input can therefore never
 // overflow it), validation flags are read after the run, and an input found
-// unsorted is redone on the eager path with a device radix sort.  That makes
+// unsorted is redone on the eager path with its oriented lists sorted on the
+// device (in place per list, or by the radix sort when a list is longer than
+// the in-place sort takes).  That makes
 // the whole device side capturable: it is recorded once per batch shape into
 // a HIP graph and replayed on later calls with the same shape and buffers.
 #include <algorithm>
@@ -114,10 +116,20 @@ static int bits_for(uint64_t x) {
     return b;
 }
 
+hipError_t batch_sort_mode(hipStream_t st, const BatchBufs& B, uint64_t N, uint64_t* h_small, uint32_t& mode) {
+    // ws_tc (tc_ws_words >= its queue header) holds the reduction's partials
+    max_len_launch((uint32_t)N, B.ooff, B.ws_tc, B.small + 8, st);
+    hipError_t e = hipMemcpyAsync(h_small + 8, B.small + 8, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    mode = h_small[8] <= segsort_max_len() ? SORT_SEGMENTED : SORT_RADIX;
+    return hipSuccess;
+}
+
 // Enqueues the device pipeline on `st` (no host synchronisation): results land
 // in h_totals (per graph) and h_small ([0] flags, [1] kept edges, [2..4] stats).
 hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
-                          uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort,
+                          uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, uint32_t sort,
                           const uint64_t sym_key[2]) {
     uint32_t* err = reinterpret_cast<uint32_t*>(B.small);
     fill_u64(B.small, 64, 0, st);
@@ -126,7 +138,9 @@ hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, cons
     OrientCheck chk{{sym_key[0], sym_key[1]}, B.vbase, G};
     orient_count_launch((uint32_t)N, nnz, B.off, B.heads, B.deg, B.ooff, B.ws_orient, err, chk, st);
     orient_write_launch((uint32_t)N, nnz, B.heads, B.oheads, B.ws_orient, st);
-    if (sort) {
+    if (sort == SORT_SEGMENTED) {
+        segsort_launch((uint32_t)N, B.ooff, B.oheads, err, segsort_max_len(), num_cus, st);
+    } else if (sort == SORT_RADIX) {
         const uint64_t m = nnz / 2;
         make_keys((uint32_t)N, m, B.ooff, B.oheads, B.keys, st);
         radix_sort_u64(B.keys, B.alt, m, 32 + bits_for(N), B.ws_radix, st);
@@ -208,13 +222,15 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
         S->h_cap = G;
     }
     hipStream_t st = c->stream;
+    uint32_t sort_mode = SORT_NONE;
     for (int attempt = 0; attempt < 2; ++attempt) {
         const bool sort = attempt == 1;
+        const bool radix = sort_mode == SORT_RADIX;
         BatchBufs B;
-        const uint64_t need = batch_layout(nullptr, G, N, nnz, host_in, !pv_dev, sort, B);
+        const uint64_t need = batch_layout(nullptr, G, N, nnz, host_in, !pv_dev, radix, B);
         rc = c->reserve(need);
         if (rc) return rc;
-        batch_layout(c->arena, G, N, nnz, host_in, !pv_dev, sort, B);
+        batch_layout(c->arena, G, N, nnz, host_in, !pv_dev, radix, B);
         if (!host_in) {
             B.off = offsets;
             B.heads = heads;
@@ -247,7 +263,7 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
                 rc = c->hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "begin capture");
                 if (rc) return rc;
                 const hipError_t le = batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals,
-                                                     c->host_small, false, c->sym_key);
+                                                     c->host_small, SORT_NONE, c->sym_key);
                 rc = c->hip_check(hipStreamEndCapture(st, &graph), "end capture");
                 if (!rc && le != hipSuccess) {
                     (void)hipGraphDestroy(graph);
@@ -276,8 +292,8 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
             S->replays++;
         } else {
             (void)hipEventRecord(c->ev[EV_PREP], st);
-            rc = c->hip_check(batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small, true,
-                                              c->sym_key),
+            rc = c->hip_check(batch_pipeline(st, c->num_cus, c->prm, B, G, N, nnz, pv, S->h_totals, c->host_small,
+                                             sort_mode, c->sym_key),
                               "batch pipeline launch");
             if (rc) return rc;
         }
@@ -296,7 +312,13 @@ extern "C" int kt_batch_count(kt_ctx* c, uint32_t G, const uint64_t* vertex_base
             return c->fail(KT_EINVAL, "malformed batch CSR (flags 0x%x, %llu oriented edges for %llu slots)", fl,
                            (unsigned long long)kept, (unsigned long long)nnz);
         }
-        if ((fl & KT_FLAG_UNSORTED) && !sort) continue;  // redo with the radix sort
+        if ((fl & KT_FLAG_UNSORTED) && !sort) {
+            // redo with the oriented lists sorted: in place, or by the radix
+            // sort when a list is longer than the in-place sort takes
+            rc = c->hip_check(batch_sort_mode(st, B, N, c->host_small, sort_mode), "longest list");
+            if (rc) return rc;
+            continue;
+        }
         memcpy(totals, S->h_totals, (uint64_t)G * 8);
         uint64_t tot = 0;
         for (uint32_t g = 0; g < G; ++g) tot += totals[g];

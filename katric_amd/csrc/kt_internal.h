@@ -167,11 +167,18 @@ struct BatchBufs {
     uint64_t* alt;
     uint64_t* ws_radix;
 };
+// How a batch pipeline sorts the oriented lists: not at all (sorted input),
+// in place per list (every list at most segsort_max_len() ids), or by the
+// global radix sort (its scratch is in the layout).
+enum BatchSort : uint32_t { SORT_NONE = 0, SORT_SEGMENTED = 1, SORT_RADIX = 2 };
 uint64_t batch_layout(void* base, uint32_t G, uint64_t N, uint64_t nnz, bool host_in, bool own_pv, bool unsorted,
                       BatchBufs& B);
 hipError_t batch_pipeline(hipStream_t st, int num_cus, const TcParams& prm, const BatchBufs& B, uint32_t G, uint64_t N,
-                          uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, bool sort,
+                          uint64_t nnz, uint64_t* pv, uint64_t* h_totals, uint64_t* h_small, uint32_t sort,
                           const uint64_t sym_key[2]);
+// After a pipeline found unsorted lists: the sort a rerun needs, from the
+// longest oriented list of that run (B as it ran; synchronises st).
+hipError_t batch_sort_mode(hipStream_t st, const BatchBufs& B, uint64_t N, uint64_t* h_small, uint32_t& mode);
 
 uint64_t tc_ws_words(uint32_t X);
 // Enqueues binning and the intersection kernels; returns the launch status.

@@ -11,7 +11,8 @@
 //
 // Every result equals the one-graph count: a batch whose validation fails is
 // re-run one request per batch so that only the malformed request fails, and
-// a batch with unsorted lists is re-run with the device radix sort.
+// a batch with unsorted lists is re-run with its oriented lists sorted on the
+// device.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -139,11 +140,11 @@ struct ServeState {
 };
 
 // Device bytes of a slot's batch: the batch pipeline's buffers (with the
-// radix-sort scratch for a relaunch of unsorted lists) and the gather
-// descriptors after them.
-static uint64_t slot_bytes(uint32_t G, uint64_t N, uint64_t nnz, bool sort) {
+// radix-sort scratch for a relaunch of unsorted lists that needs it) and the
+// gather descriptors after them.
+static uint64_t slot_bytes(uint32_t G, uint64_t N, uint64_t nnz, bool radix) {
     BatchBufs B;
-    return ((batch_layout(nullptr, G, N, nnz, true, true, sort, B) + 255) & ~255ull) + (uint64_t)G * sizeof(GatherDesc);
+    return ((batch_layout(nullptr, G, N, nnz, true, true, radix, B) + 255) & ~255ull) + (uint64_t)G * sizeof(GatherDesc);
 }
 
 static uint64_t req_bytes(const ServeReq* r) { return slot_bytes(1, r->n, r->nnz, false); }
@@ -178,7 +179,7 @@ static int slot_grow(ServeSlot& s, uint64_t bytes, uint32_t G) {
 }
 
 // Uploads, rebases and enqueues one batch on the slot (worker thread, no lock).
-static int launch(ServeState* S, ServeSlot& sl, bool sort) {
+static int launch(ServeState* S, ServeSlot& sl, uint32_t sort) {
     const uint32_t G = (uint32_t)sl.reqs.size();
     uint64_t N = 0, nnz = 0;
     for (auto* r : sl.reqs) {
@@ -186,13 +187,14 @@ static int launch(ServeState* S, ServeSlot& sl, bool sort) {
         nnz += r->nnz;
     }
     BatchBufs B;
-    const uint64_t used = (batch_layout(nullptr, G, N, nnz, true, true, sort, B) + 255) & ~255ull;
+    const bool radix = sort == SORT_RADIX;
+    const uint64_t used = (batch_layout(nullptr, G, N, nnz, true, true, radix, B) + 255) & ~255ull;
     // a slot never holds more than its share of the budget (a relaunch with
-    // the sort scratch can need more than admission counted)
-    if (slot_bytes(G, N, nnz, sort) > S->slot_cap) return KT_ENOMEM;
+    // the radix-sort scratch can need more than admission counted)
+    if (slot_bytes(G, N, nnz, radix) > S->slot_cap) return KT_ENOMEM;
     int rc = slot_grow(sl, used + (uint64_t)G * sizeof(GatherDesc), G);
     if (rc) return rc;
-    batch_layout(sl.arena, G, N, nnz, true, true, sort, B);
+    batch_layout(sl.arena, G, N, nnz, true, true, radix, B);
     GatherDesc* d_desc = reinterpret_cast<GatherDesc*>(static_cast<char*>(sl.arena) + used);
     uint64_t* vb = sl.h_meta;
     GatherDesc* desc = reinterpret_cast<GatherDesc*>(sl.h_meta + (sl.meta_cap + 1));
@@ -241,7 +243,7 @@ static int launch(ServeState* S, ServeSlot& sl, bool sort) {
     if (hipEventRecord(sl.ev, st) != hipSuccess) return KT_EHIP;
     sl.N = N;
     sl.nnz = nnz;
-    sl.sorted = sort;
+    sl.sorted = sort != SORT_NONE;
     sl.busy = true;
     return KT_OK;
 }
@@ -316,7 +318,13 @@ static void worker_main(ServeState* S) {
             const uint32_t fl = (uint32_t)sl.h_small[0];
             if (!(fl & KT_ERR_MASK) && (fl & KT_FLAG_UNSORTED) && !sl.sorted && 2 * sl.h_small[1] == sl.nnz) {
                 lk.unlock();
-                int rc = launch(S, sl, true);  // same requests, with the radix sort
+                // same requests, with the oriented lists sorted (the sort
+                // chosen from the longest list of the run that found them)
+                BatchBufs B0;
+                batch_layout(sl.arena, (uint32_t)sl.reqs.size(), sl.N, sl.nnz, true, true, false, B0);
+                uint32_t mode = SORT_RADIX;
+                int rc = batch_sort_mode(sl.stream, B0, sl.N, sl.h_small, mode) == hipSuccess ? KT_OK : KT_EHIP;
+                if (rc == KT_OK) rc = launch(S, sl, mode);
                 lk.lock();
                 if (rc != KT_OK) launch_failed(S, sl, rc);
                 continue;
@@ -396,7 +404,7 @@ static void worker_main(ServeState* S) {
                 free_slot->large = blocked_large;
                 free_slot->batch_id = S->next_batch++;
                 lk.unlock();
-                int rc = launch(S, *free_slot, false);
+                int rc = launch(S, *free_slot, SORT_NONE);
                 lk.lock();
                 if (rc != KT_OK) launch_failed(S, *free_slot, rc);
                 continue;

@@ -67,6 +67,26 @@ def test_batch_unsorted_and_empty_graphs(ctx):
     assert t1[1] == 0
 
 
+def test_batch_unsorted_long_list(ctx):
+    # one graph's longest oriented list (4098 ids) is beyond the in-place
+    # sort: the batch is redone with the radix sort; K_n minus a perfect
+    # matching has C(n,3) - (n/2)(n-2) triangles
+    n = 4100
+    iu, ju = np.triu_indices(n, 1)
+    keep = ~((iu == 0) & (ju == 2)) & ~((iu == 1) & (ju == 3)) & ~((iu >= 4) & (iu % 2 == 0) & (ju == iu + 1))
+    big = K.Graph.from_edges(n, np.stack([iu[keep], ju[keep]], 1))
+    graphs = [K.gen_gnm(300, 2000, seed=5), big, K.gen_rmat(9, 16, seed=6)]
+    b = KB.from_graphs(graphs)
+    rng = np.random.default_rng(3)
+    heads = b.heads.copy()
+    for v in range(b.n_total):
+        a, e = int(b.offsets[v]), int(b.offsets[v + 1])
+        heads[a:e] = rng.permutation(heads[a:e])
+    tot = KB.count_batch(ctx, KB.Batch(b.vertex_base, b.offsets, heads))
+    assert tot[1] == n * (n - 1) * (n - 2) // 6 - (n // 2) * (n - 2)
+    assert tot[0] == ctx.count(graphs[0])[0] and tot[2] == ctx.count(graphs[2])[0]
+
+
 def test_batch_totals_across_chunk_boundaries(ctx):
     # per-graph totals are summed in 512-vertex chunks: graphs with no vertices,
     # graphs that end exactly on a chunk boundary and graphs spanning many chunks

@@ -150,11 +150,9 @@ def test_serve_page_locked_malformed_isolated(ctx):
 
 
 def test_serve_unsorted_under_tight_budget(ctx):
-    # unsorted requests: a batch admitted without the radix-sort scratch is
-    # re-launched with it; when that no longer fits the slot's half of the
-    # budget, its requests run one at a time instead of failing together.
-    # Several budgets around the point where that happens: every result must
-    # equal the oracle, and at least one budget must have isolated a batch.
+    # unsorted requests: a batch that finds unsorted lists is re-launched with
+    # them sorted in place (no scratch beyond the first launch's), under
+    # budgets from a few MiB up: every result must equal the oracle
     from oracle import oracle as O
 
     rng = np.random.default_rng(4)
@@ -179,7 +177,40 @@ def test_serve_unsorted_under_tight_budget(ctx):
         for (t, opv), r, pv in zip(want, res, pvs):
             assert r.status == 0 and r.total == t, (mib, r.status)
             np.testing.assert_array_equal(pv, opv)
-    assert isolated > 0
+
+
+def _clique_minus_matching_shuffled(n, seed):
+    """K_n minus the matching (0,2), (1,3), (4,5), ...: all degrees n - 2,
+    vertex 0's oriented list n - 2 ids; rows shuffled.  Triangles:
+    C(n,3) - (n/2)(n-2)."""
+    iu, ju = np.triu_indices(n, 1)
+    keep = ~((iu == 0) & (ju == 2)) & ~((iu == 1) & (ju == 3)) & ~((iu >= 4) & (iu % 2 == 0) & (ju == iu + 1))
+    g = K.Graph.from_edges(n, np.stack([iu[keep], ju[keep]], 1))
+    rng = np.random.default_rng(seed)
+    heads = g.heads.copy()
+    for v in range(g.n):
+        a, b = int(g.offsets[v]), int(g.offsets[v + 1])
+        heads[a:b] = rng.permutation(heads[a:b])
+    return K.Graph(g.n, g.offsets, heads)
+
+
+def test_serve_unsorted_long_lists(ctx):
+    # a request whose longest oriented list (4098 ids) is beyond the in-place
+    # sort: its batch is re-launched with the radix sort; small requests
+    # queued around it are answered as usual
+    from oracle import oracle as O
+
+    n = 4100
+    big = _clique_minus_matching_shuffled(n, 7)
+    small = [K.gen_gnm(300, 2000, seed=60 + s) for s in range(3)]
+    with K.Context(0) as sctx:
+        srv = KS.Server(sctx)
+        tickets = [srv.submit(g) for g in [small[0], big] + small[1:]]
+        res = [srv.wait(t) for t in tickets]
+        srv.stop()
+    assert res[1].status == 0 and res[1].total == n * (n - 1) * (n - 2) // 6 - (n // 2) * (n - 2)
+    for r, g in zip([res[0]] + res[2:], small):
+        assert r.status == 0 and r.total == O.run_shmem(g)[0]
 
 
 def test_serve_large_requests_run_alone(ctx):
