@@ -307,6 +307,7 @@ struct PackLds {
 struct Lds {
     uint32_t keys[GROUP_SLOTS];
     uint32_t cnt[GROUP_SLOTS];
+    uint32_t ovf[GROUP_SLOTS / 128];  // a bit per bucket (the team's table, or 4 words per wave table)
     PackLds pk[TC_WAVES];
     uint64_t red[TC_WAVES];
     unsigned long long list_ctr[TC_WAVES];  // listing: triangles of the current tail written so far
@@ -317,6 +318,7 @@ struct Lds {
 
 // Team control words of k_tc_big (its table is dynamic LDS).
 struct LdsBig {
+    uint32_t ovf[BIG_SLOTS / 128];  // a bit per bucket of the dynamic-LDS table
     PackLds pk[BIG_WAVES];
     uint64_t red[BIG_WAVES];
     unsigned long long list_ctr[BIG_WAVES];
@@ -355,19 +357,22 @@ __device__ __forceinline__ const uint4* bucket_at(const uint32_t* keys, uint32_t
     return reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(keys) + off);
 }
 
-// Inserts w; returns true when w took the last slot of a bucket or went past
-// its home bucket: a table where no insertion did holds no full bucket, and
-// every lookup there ends in the home bucket.
-__device__ __forceinline__ bool table_insert(uint32_t* keys, uint32_t boff_mask, uint32_t w) {
-    uint32_t b = bucket_off(w, boff_mask);
+// Inserts w; returns true when w went past its home bucket (which then gets
+// its bit in `ovf`, one bit per bucket).  A lookup continues past its home
+// bucket only when that bucket is full and has the bit; in a table where no
+// insertion went past its home bucket, every lookup ends there.
+__device__ __forceinline__ bool table_insert(uint32_t* keys, uint32_t* ovf, uint32_t boff_mask, uint32_t w) {
+    const uint32_t home = bucket_off(w, boff_mask);
+    uint32_t b = home;
     bool moved = false;
     while (true) {
         uint32_t* bk = at_byte(keys, b);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             uint32_t old = atomicCAS(&bk[s], KT_EMPTY, w);
-            if (old == KT_EMPTY || old == w) return moved || s == 3;
+            if (old == KT_EMPTY || old == w) return moved;
         }
+        if (!moved) atomicOr(&ovf[home >> 9], 1u << ((home >> 4) & 31));
         b = (b + 16) & boff_mask;
         moved = true;
     }
@@ -377,12 +382,13 @@ __device__ __forceinline__ bool table_insert(uint32_t* keys, uint32_t boff_mask,
 // lane's number of hits.  Each hit bumps the w-counter of its slot and calls
 // on_hit(e).  The four slot compares of a probe stay lane masks; hits are
 // counted per lane (one add), so the scalar unit only sees the branch masks.
-// Lookups continue past a full home bucket only when the table has one
-// (full_buckets, wave-uniform).
+// Lookups continue past a full home bucket only in a table where some key
+// went past its home bucket (moved, wave-uniform), and only from a bucket
+// with its `ovf` bit.
 template <class OnHit>
-__device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, uint32_t boff_mask,
-                                           bool full_buckets, const uint32_t (&w)[4], const bool (&valid)[4],
-                                           OnHit on_hit) {
+__device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, const uint32_t* ovf,
+                                           uint32_t boff_mask, bool moved, const uint32_t (&w)[4],
+                                           const bool (&valid)[4], OnHit on_hit) {
     uint32_t bo[4];
     uint4 kk[4];
 #pragma unroll
@@ -402,7 +408,7 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
             on_hit(e);
         }
     }
-    if (!full_buckets) return nh;
+    if (!moved) return nh;
     // a full home bucket without w: w may sit in a later bucket
     bool need[4];
 #pragma unroll
@@ -412,6 +418,8 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             if (!need[e]) continue;
+            // nothing went past w's home bucket: w is not in the table
+            if (!((ovf[bo[e] >> 9] >> ((bo[e] >> 4) & 31)) & 1u)) continue;
             uint32_t b = bo[e];
             while (true) {
                 b = (b + 16) & boff_mask;
@@ -501,8 +509,9 @@ struct ProbeCtx {
     Emitter em;  // listing launches only
     uint32_t* keys;
     uint32_t* cnt;
+    uint32_t* ovf;   // one bit per bucket: some key went past it
     uint32_t boff;   // (buckets - 1) << 4: bucket byte-offset mask
-    bool full;       // some bucket is full (wave-uniform)
+    bool full;       // some key went past its home bucket (wave-uniform)
     const uint32_t* s;  // sorted S(x) (binary-search probes)
     uint64_t ns;
     uint64_t* pv;
@@ -581,9 +590,9 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
         if (HASH) {
             if constexpr (LIST)
-                nh += probe4(P.keys, P.cnt, P.boff, P.full, w, valid, [&](int e) { P.em(u, w[e]); });
+                nh += probe4(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, [&](int e) { P.em(u, w[e]); });
             else
-                nh += probe4(P.keys, P.cnt, P.boff, P.full, w, valid, NoHitAction{});
+                nh += probe4(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, NoHitAction{});
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -686,7 +695,7 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
         const int lim = (int)clen - (int)(pos + s4);
         const bool valid[4] = {0 < lim, 1 < lim, 2 < lim, 3 < lim};
         const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
-        nh += probe4(P.keys, P.cnt, P.boff, P.full, w, valid, [&](int) { atomicAdd(&pk.ucnt[csrc], 1u); });
+        nh += probe4(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, [&](int) { atomicAdd(&pk.ucnt[csrc], 1u); });
         // advance; a group that switched lists fetches its list after next
         uint32_t fsrc, flen;
         const uint32_t* fbase;
@@ -727,8 +736,8 @@ __device__ __forceinline__ TailBounds tail_bounds(const View<PLAIN>& V, uint32_t
 // this wave alone).
 template <int NW, bool HASH, bool PLAIN, int GL, bool LIST, class LdsT>
 __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, const TailBounds& tb, LdsT& L,
-                                             uint32_t* keys, uint32_t* cnt, uint32_t cap_slots, uint32_t shift,
-                                             uint32_t chunk, const TcOut& out,
+                                             uint32_t* keys, uint32_t* cnt, uint32_t* ovf, uint32_t cap_slots,
+                                             uint32_t shift, uint32_t chunk, const TcOut& out,
                                              uint64_t& my_total, WorkStats& wk) {
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x / KT_WAVE;
@@ -743,6 +752,7 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, c
     ProbeCtx P;
     P.keys = keys;
     P.cnt = cnt;
+    P.ovf = ovf;
     P.boff = 0;
     P.full = false;
     P.s = V.v.sheads + sb;
@@ -762,9 +772,11 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, c
             keys[i] = KT_EMPTY;
             cnt[i] = 0;
         }
+        for (uint32_t i = tid; i < (slots + 127) / 128; i += NW * KT_WAVE) ovf[i] = 0;  // one bit per 4-slot bucket
         team_sync<NW>();
         bool full = false;
-        for (uint64_t i = tid; i < ns; i += NW * KT_WAVE) full |= table_insert(keys, P.boff, V.v.sheads[sb + i]);
+        for (uint64_t i = tid; i < ns; i += NW * KT_WAVE)
+            full |= table_insert(keys, ovf, P.boff, V.v.sheads[sb + i]);
         if (NW > 1) {
             if (tid == 0) L.full = 0;
             team_sync<NW>();
@@ -873,7 +885,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
             __syncthreads();
             if (i >= cnt) break;
             const uint32_t x = list[base + i];
-            team_process<TC_WAVES, true, PLAIN, GL, LIST>(V, x, tail_bounds(V, x), L, L.keys, L.cnt, GROUP_SLOTS,
+            team_process<TC_WAVES, true, PLAIN, GL, LIST>(V, x, tail_bounds(V, x), L, L.keys, L.cnt, L.ovf, GROUP_SLOTS,
                                                           prm.table_shift, prm.team_chunk, out, my_total, wk);
         }
     }
@@ -914,7 +926,8 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
                 for (uint32_t j = 0; j < ng; ++j) {
                     const TailBounds tb{readlane64(bl.sb, j), readlane64(bl.se, j), readlane64(bl.rb, j),
                                         readlane64(bl.re, j), readlane64(bl.nstat, j)};
-                    team_process<1, true, PLAIN, GL, LIST>(V, readlane32(xl, j), tb, L, keys, tcnt, WAVE_SLOTS,
+                    team_process<1, true, PLAIN, GL, LIST>(V, readlane32(xl, j), tb, L, keys, tcnt,
+                                                           L.ovf + wave * (WAVE_SLOTS / 128), WAVE_SLOTS,
                                                            prm.table_shift, KT_WAVE, out, my_total, wk);
                 }
             }
@@ -970,7 +983,7 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
         __syncthreads();
         if (i >= cnt) break;
         const uint32_t x = list[base + i];
-        team_process<TC_WAVES, false, PLAIN, KT_WAVE, LIST>(V, x, tail_bounds(V, x), L, nullptr, nullptr, 0, 0,
+        team_process<TC_WAVES, false, PLAIN, KT_WAVE, LIST>(V, x, tail_bounds(V, x), L, nullptr, nullptr, nullptr, 0, 0,
                                                             prm.team_chunk, out, my_total, wk);
     }
     if (lane_id() == 0) {
@@ -1006,7 +1019,7 @@ __global__ void __launch_bounds__(BIG_THREADS) k_tc_big(View<PLAIN> V, TcOut out
         if (i >= cnt) break;
         const uint32_t x = list[base + i];
         team_process<BIG_WAVES, true, PLAIN, KT_WAVE, LIST>(V, x, tail_bounds(V, x), L, big_table,
-                                                            big_table + BIG_SLOTS, BIG_SLOTS, prm.table_shift,
+                                                            big_table + BIG_SLOTS, L.ovf, BIG_SLOTS, prm.table_shift,
                                                             prm.team_chunk, out, my_total, wk);
     }
     if (lane_id() == 0) {
