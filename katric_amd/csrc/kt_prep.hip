@@ -169,21 +169,11 @@ __global__ void k_tile_bounds(uint32_t n, uint64_t nnz, uint64_t tile, const uin
 }
 
 // Graph bounds of a tile's rows for the batch range check: the graphs of its
-// first and last row, and the vertex range of the first.
+// first and last row (from k_tile_graphs), and the vertex range of the first.
 struct TileGraphs {
     uint32_t glo, ghi;
     uint64_t a, b;
 };
-
-__device__ __forceinline__ TileGraphs tile_graphs(const uint64_t* __restrict__ vbase, uint32_t G, uint32_t lo,
-                                                  uint32_t hi) {
-    TileGraphs t;
-    t.glo = graph_of(vbase, 0, G - 1, lo);
-    t.ghi = graph_of(vbase, t.glo, G - 1, hi);
-    t.a = vbase[t.glo];
-    t.b = vbase[t.glo + 1];
-    return t;
-}
 
 // A head u of row v lies outside v's graph.
 __device__ __forceinline__ bool outside_graph(const uint64_t* __restrict__ vbase, const TileGraphs& t, uint32_t v,
@@ -210,7 +200,22 @@ struct CheckArgs {
     uint64_t k0, k1;
     const uint64_t* vbase;
     uint32_t G;
+    const uint32_t* tgraph;  // batch: graphs of each tile's first and last row (k_tile_graphs)
 };
+
+// Graphs of every tile's first and last row, for the batch range check: one
+// thread per tile, so the binary searches over vertex_base run side by side
+// instead of inside each tile's block.  Rows clamped as tile_rows clamps them.
+__global__ void k_tile_graphs(uint32_t n, uint64_t tiles, const uint32_t* __restrict__ tbnd,
+                              const uint64_t* __restrict__ vbase, uint32_t G, uint32_t* __restrict__ tg) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles) return;
+    const uint32_t a = min(tbnd[2 * t], n - 1);
+    const uint32_t b = max(a, min(tbnd[2 * t + 1], n - 1));
+    const uint32_t glo = graph_of(vbase, 0, G - 1, a);
+    tg[2 * t] = glo;
+    tg[2 * t + 1] = graph_of(vbase, glo, G - 1, b);
+}
 
 template <bool BATCH>
 __global__ void __launch_bounds__(OR_THREADS)
@@ -235,7 +240,10 @@ __global__ void __launch_bounds__(OR_THREADS)
     const uint64_t nrows = (uint64_t)hi - lo + 1;
     const bool lds_rows = nrows <= OR_TILE;
     TileGraphs tg{};
-    if (BATCH) tg = tile_graphs(ck.vbase, ck.G, lo, hi);
+    if (BATCH) {
+        const uint32_t glo = ck.tgraph[2 * blockIdx.x];
+        tg = TileGraphs{glo, ck.tgraph[2 * blockIdx.x + 1], ck.vbase[glo], ck.vbase[glo + 1]};
+    }
     uint32_t flags = 0, cnt = 0;
     // The 64 slots of a wave are consecutive: the next slot's head and row
     // come from the next lane (the last lane loads them), and kept slots are
@@ -871,6 +879,7 @@ struct OrientWs {
     uint32_t* tbnd;      // 2 * tiles: rows of each tile's first and last slot
     uint64_t* stmp;
     uint8_t* ncls;       // (n + 1) / 2: four-bit degree classes
+    uint32_t* tgraph;    // 2 * tiles: batch, graphs of each tile's first and last row
 };
 
 static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
@@ -883,8 +892,9 @@ static OrientWs orient_ws(uint32_t n, uint64_t nnz, uint64_t* ws) {
     w.dplus = w.bits + (nnz + 63) / 64 + 1;
     w.tbnd = reinterpret_cast<uint32_t*>(w.dplus + (uint64_t)n + 1);
     w.stmp = w.dplus + (uint64_t)n + 1 + w.tiles;
-    w.ncls = reinterpret_cast<uint8_t*>(
-        w.stmp + std::max(scan_tmp_words(w.tiles + 1), scan_tmp_words((uint64_t)n + 1)));
+    uint64_t* after = w.stmp + std::max(scan_tmp_words(w.tiles + 1), scan_tmp_words((uint64_t)n + 1));
+    w.ncls = reinterpret_cast<uint8_t*>(after);
+    w.tgraph = reinterpret_cast<uint32_t*>(after + ((uint64_t)n + 15) / 16);
     return w;
 }
 
@@ -895,7 +905,7 @@ const uint64_t* orient_keep_bits(uint32_t n, uint64_t nnz, const uint64_t* ws) {
 uint64_t orient_ws_words(uint32_t n, uint64_t nnz) {
     uint64_t tiles = (nnz + OR_TILE - 1) / OR_TILE;
     return (tiles + 1) + 2 * (tiles + 1) + (nnz + 63) / 64 + 1 + (uint64_t)n + 1 + tiles +
-           std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1)) + ((uint64_t)n + 15) / 16;
+           std::max(scan_tmp_words(tiles + 1), scan_tmp_words((uint64_t)n + 1)) + ((uint64_t)n + 15) / 16 + tiles;
 }
 
 // Rows of the first and last slot of every `tile`-slot tile of a CSR whose
@@ -915,15 +925,18 @@ void orient_count_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const ui
     k_degree<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, off, deg, w.ncls, err);
     fill_u64(w.dplus, (uint64_t)n + 1, 0, st);
     if (w.tiles > 0) {
-        const CheckArgs ck{chk.key[0], chk.key[1], chk.G ? chk.vbase : nullptr, chk.G};
+        const CheckArgs ck{chk.key[0], chk.key[1], chk.G ? chk.vbase : nullptr, chk.G, w.tgraph};
         fill_u32(w.tbnd, 2 * w.tiles, 0, st);
         k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, w.tbnd);
-        if (ck.vbase)
+        if (ck.vbase) {
+            k_tile_graphs<<<(unsigned)((w.tiles + 255) / 256), 256, 0, st>>>(n, w.tiles, w.tbnd, ck.vbase, ck.G,
+                                                                            w.tgraph);
             k_orient_keep<true><<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.ncls, w.tbnd,
                                                                          w.bits, w.dplus, w.tile_cnt, err, ck, w.sym);
-        else
+        } else {
             k_orient_keep<false><<<(unsigned)w.tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, deg, w.ncls, w.tbnd,
                                                                           w.bits, w.dplus, w.tile_cnt, err, ck, w.sym);
+        }
         scan_u64(w.tile_cnt, w.tile_off, w.tiles, w.stmp, st);
     } else {
         fill_u64(w.tile_off, 1, 0, st);
@@ -944,7 +957,7 @@ void sym_check_launch(uint32_t n, uint64_t nnz, const uint64_t* off, const uint3
     uint64_t* sym = ws;
     uint32_t* tbnd = reinterpret_cast<uint32_t*>(ws + tiles + 1);
     if (tiles > 0 && n > 0) {
-        const CheckArgs ck{chk.key[0], chk.key[1], nullptr, 0};
+        const CheckArgs ck{chk.key[0], chk.key[1], nullptr, 0, nullptr};
         fill_u32(tbnd, 2 * tiles, 0, st);
         k_tile_bounds<<<(n + 255) / 256, 256, 0, st>>>(n, nnz, OR_TILE, off, tbnd);
         k_sym_tiles<<<(unsigned)tiles, OR_THREADS, 0, st>>>(n, nnz, off, heads, tbnd, err, ck, sym);

@@ -503,6 +503,9 @@ struct Emitter {
 // intersection_size, and the T-list ids this design streams through probes.
 struct WorkStats {
     uint64_t wedge = 0, isz = 0, stream = 0;
+    uint64_t stream_lane = 0;  // per lane, summed into stream once per wave at the end (grouped walks)
+    // the wave's total of stream and stream_lane (every lane calls this)
+    __device__ __forceinline__ uint64_t streamed() const { return stream + uniform64(wave_sum(stream_lane)); }
 };
 
 struct ProbeCtx {
@@ -650,7 +653,7 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
         const uint64_t cw = uniform64(wave_sum(stl));
         wk.wedge += cw;
         wk.isz += cw + (uint64_t)__popcll(__builtin_amdgcn_ballot_w64(stc)) * nstat;
-        wk.stream += uniform64(wave_sum(nonempty ? tej - tbj : 0ull));
+        wk.stream_lane += nonempty ? tej - tbj : 0ull;
     }
     if (nne == 0) return 0;
     // order[r] = lane of the r-th non-empty list
@@ -934,6 +937,7 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
         }
     }
     // flush the wave-uniform accumulators: one atomic per workgroup and counter
+    const uint64_t streamed = wk.streamed();
     __syncthreads();
     if (lane == 0) {
         L.red[wave] = my_total;
@@ -941,8 +945,8 @@ __global__ void __launch_bounds__(TC_THREADS) __attribute__((amdgpu_waves_per_eu
         L.keys[TC_WAVES + wave] = (uint32_t)(wk.wedge >> 32);
         L.cnt[wave] = (uint32_t)wk.isz;
         L.cnt[TC_WAVES + wave] = (uint32_t)(wk.isz >> 32);
-        L.keys[2 * TC_WAVES + wave] = (uint32_t)wk.stream;
-        L.keys[3 * TC_WAVES + wave] = (uint32_t)(wk.stream >> 32);
+        L.keys[2 * TC_WAVES + wave] = (uint32_t)streamed;
+        L.keys[3 * TC_WAVES + wave] = (uint32_t)(streamed >> 32);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -986,11 +990,12 @@ __global__ void __launch_bounds__(TC_THREADS) k_tc_huge(View<PLAIN> V, TcOut out
         team_process<TC_WAVES, false, PLAIN, KT_WAVE, LIST>(V, x, tail_bounds(V, x), L, nullptr, nullptr, nullptr, 0, 0,
                                                             prm.team_chunk, out, my_total, wk);
     }
+    const uint64_t streamed = wk.streamed();
     if (lane_id() == 0) {
         if (my_total) atomic_add_u64(&out.totals[0], my_total);
         if (wk.wedge) atomic_add_u64(&stats[ST_WEDGE], wk.wedge);
         if (wk.isz) atomic_add_u64(&stats[ST_ISZ], wk.isz);
-        if (wk.stream) atomic_add_u64(&stats[ST_STREAM], wk.stream);
+        if (streamed) atomic_add_u64(&stats[ST_STREAM], streamed);
     }
 }
 
@@ -1022,11 +1027,12 @@ __global__ void __launch_bounds__(BIG_THREADS) k_tc_big(View<PLAIN> V, TcOut out
                                                             big_table + BIG_SLOTS, L.ovf, BIG_SLOTS, prm.table_shift,
                                                             prm.team_chunk, out, my_total, wk);
     }
+    const uint64_t streamed = wk.streamed();
     if (lane_id() == 0) {
         if (my_total) atomic_add_u64(&out.totals[0], my_total);
         if (wk.wedge) atomic_add_u64(&stats[ST_WEDGE], wk.wedge);
         if (wk.isz) atomic_add_u64(&stats[ST_ISZ], wk.isz);
-        if (wk.stream) atomic_add_u64(&stats[ST_STREAM], wk.stream);
+        if (streamed) atomic_add_u64(&stats[ST_STREAM], streamed);
     }
 }
 
