@@ -253,11 +253,14 @@ __global__ void __launch_bounds__(OR_THREADS)
     // from an n/2-byte table, half the L2 footprint of byte classes, and the
     // rows' degrees), so that a thread has every item's memory traffic in
     // flight at once
-    uint32_t hu[OR_ITEMS], cu[OR_ITEMS], dv[OR_ITEMS];
+    // (and, on a wave's last lane, the head of the slot after it, which the
+    // adjacency check of that lane compares against)
+    uint32_t hu[OR_ITEMS], hn[OR_ITEMS], cu[OR_ITEMS], dv[OR_ITEMS];
 #pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
         const uint64_t sl = tile0 + k * OR_THREADS + tid;
         hu[k] = sl < tend ? heads[sl] : 0u;
+        hn[k] = (lane == KT_WAVE - 1 && sl + 1 < nnz) ? heads[sl + 1] : 0u;
     }
 #pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
@@ -306,7 +309,7 @@ __global__ void __launch_bounds__(OR_THREADS)
         if (lane == KT_WAVE - 1) {
             same = in && ((i + 1 < OR_TILE) ? (sl + 1 < tend && marks[i + 1] == rr)
                                             : (sl + 1 < nnz && sl + 1 < off[v + 1]));
-            if (same) nx = heads[sl + 1];
+            nx = hn[k];
         }
         if (in && same) {
             if (nx == u)
@@ -443,9 +446,11 @@ __global__ void __launch_bounds__(OR_THREADS)
 }
 
 // Copies the kept heads in slot order: output position = tile base + rank of
-// the slot among the tile's kept slots.  Every wave loads its 16 bitmask
-// words first; one barrier publishes the per-(round, wave) counts, from which
-// each wave derives all its output positions.
+// the slot among the tile's kept slots.  Every wave loads its bitmask words
+// and all its heads first (coalesced whole rows of 64 slots; the dropped
+// heads share their cache lines with the kept ones), so the loads are in
+// flight together; one barrier publishes the per-(round, wave) counts, from
+// which each wave derives all its output positions.
 __global__ void __launch_bounds__(OR_THREADS)
     k_orient_emit(uint64_t nnz, const uint32_t* __restrict__ heads, const uint64_t* __restrict__ keep_bits,
                   const uint64_t* __restrict__ tile_off, uint32_t* __restrict__ oheads) {
@@ -454,14 +459,18 @@ __global__ void __launch_bounds__(OR_THREADS)
     const uint64_t tile0 = (uint64_t)blockIdx.x * OR_TILE;
     const unsigned tid = threadIdx.x, wave = tid / KT_WAVE, lane = lane_id();
     uint64_t bits[OR_ITEMS];
+    uint32_t hv[OR_ITEMS];
 #pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
         const uint64_t w0 = tile0 + k * OR_THREADS + wave * KT_WAVE;
         bits[k] = (w0 < nnz) ? keep_bits[w0 / 64] : 0;
-        if (lane == 0) wcnt[k][wave] = (uint32_t)__popcll(bits[k]);
+        hv[k] = (w0 + lane < nnz) ? heads[w0 + lane] : 0u;
     }
-    __syncthreads();
     uint64_t base = tile_off[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < OR_ITEMS; ++k)
+        if (lane == 0) wcnt[k][wave] = (uint32_t)__popcll(bits[k]);
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < OR_ITEMS; ++k) {
         uint32_t before = 0, round = 0;
@@ -471,10 +480,7 @@ __global__ void __launch_bounds__(OR_THREADS)
             before += (w < wave) ? c : 0;
             round += c;
         }
-        if ((bits[k] >> lane) & 1) {
-            const uint64_t w0 = tile0 + k * OR_THREADS + wave * KT_WAVE;
-            oheads[base + before + __popcll(bits[k] & lanemask_lt())] = heads[w0 + lane];
-        }
+        if ((bits[k] >> lane) & 1) oheads[base + before + __popcll(bits[k] & lanemask_lt())] = hv[k];
         base += round;
     }
 }
