@@ -43,14 +43,16 @@ __device__ __forceinline__ uint64_t block_excl_sum(uint64_t x, uint64_t* lds_wav
     return base + incl - x;
 }
 
+// The tile's sum: the order of the terms does not matter, so the loads are
+// coalesced (element tile0 + i * 256 + tid), unlike the scan's per-thread runs.
 __global__ void __launch_bounds__(SCAN_THREADS) k_tile_reduce(const uint64_t* __restrict__ in, uint64_t len,
                                                                uint64_t* __restrict__ sums) {
     __shared__ uint64_t lw[SCAN_THREADS / KT_WAVE];
-    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
     uint64_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i)
-        if (base + i < len) s += in[base + i];
+        if (base + i * SCAN_THREADS < len) s += in[base + i * SCAN_THREADS];
     uint64_t tot;
     block_excl_sum(s, lw, &tot);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
