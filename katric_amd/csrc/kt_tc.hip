@@ -751,6 +751,13 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, c
     const uint64_t rb = uniform64(tb.rb), re = uniform64(tb.re);
     const uint64_t ns = se - sb, nr = re - rb;
     const uint64_t nstat = uniform64(tb.nstat);
+    // The first chunk's neighbours are loaded before the tail's table is
+    // built (clearing it and inserting S(x) hide the round trip); every later
+    // chunk's neighbours are loaded while the chunk before it is scanned (the
+    // bounds of their T lists depend on them).
+    uint64_t k = (uint64_t)twave * chunk;
+    uint32_t u_nx = 0, kw_nx = 0;
+    if (k < nr && lane < min<uint64_t>(chunk, nr - k)) V.entry_head(rb + k + lane, u_nx, kw_nx);
 
     ProbeCtx P;
     P.keys = keys;
@@ -798,11 +805,6 @@ __device__ __forceinline__ void team_process(const View<PLAIN>& V, uint32_t x, c
     team_sync<NW>();
 
     uint64_t tail_hits = 0;
-    uint64_t k = (uint64_t)twave * chunk;
-    // the neighbours of the next chunk are loaded while the current one is
-    // scanned (the bounds of their T lists depend on them)
-    uint32_t u_nx = 0, kw_nx = 0;
-    if (k < nr && lane < min<uint64_t>(chunk, nr - k)) V.entry_head(rb + k + lane, u_nx, kw_nx);
     while (k < nr) {
         const uint32_t c = (uint32_t)min<uint64_t>(chunk, nr - k);
         const uint32_t u_cur = u_nx, kw_cur = kw_nx;
