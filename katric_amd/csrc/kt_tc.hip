@@ -378,28 +378,28 @@ __device__ __forceinline__ bool table_insert(uint32_t* keys, uint32_t* ovf, uint
     }
 }
 
-// Probes up to 4 ids of this lane (id e valid iff valid[e]) and returns this
-// lane's number of hits.  Each hit bumps the w-counter of its slot and calls
+// Probes the first N (2 or 4) ids of this lane (id e valid iff valid[e]) and
+// returns this lane's number of hits.  Each hit bumps the w-counter of its slot and calls
 // on_hit(e).  The four slot compares of a probe stay lane masks; hits are
 // counted per lane (one add), so the scalar unit only sees the branch masks.
 // Lookups continue past a full home bucket only in a table where some key
 // went past its home bucket (moved, wave-uniform), and only from a bucket
 // with its `ovf` bit.
-template <class OnHit>
-__device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, const uint32_t* ovf,
+template <int N, class OnHit>
+__device__ __forceinline__ uint32_t probe_ids(const uint32_t* keys, uint32_t* cnt, const uint32_t* ovf,
                                            uint32_t boff_mask, bool moved, const uint32_t (&w)[4],
                                            const bool (&valid)[4], OnHit on_hit) {
     uint32_t bo[4];
     uint4 kk[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < N; ++e) {
         bo[e] = bucket_off(w[e], boff_mask);
         kk[e] = *bucket_at(keys, bo[e]);
     }
     uint32_t nh = 0;
     bool hit[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < N; ++e) {
         const bool m0 = kk[e].x == w[e], m1 = kk[e].y == w[e], m2 = kk[e].z == w[e], m3 = kk[e].w == w[e];
         hit[e] = valid[e] && (m0 || m1 || m2 || m3);
         nh += hit[e] ? 1u : 0u;
@@ -412,11 +412,14 @@ __device__ __forceinline__ uint32_t probe4(const uint32_t* keys, uint32_t* cnt, 
     // a full home bucket without w: w may sit in a later bucket
     bool need[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) need[e] = valid[e] && !hit[e] && kk[e].w != KT_EMPTY;
-    // one branch for the four ids, skipped as a whole when no lane needs it
-    if (need[0] || need[1] || need[2] || need[3]) {
+    for (int e = 0; e < N; ++e) need[e] = valid[e] && !hit[e] && kk[e].w != KT_EMPTY;
+    // one branch for the N ids, skipped as a whole when no lane needs it
+    bool any = false;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < N; ++e) any = any || need[e];
+    if (any) {
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
             if (!need[e]) continue;
             // nothing went past w's home bucket: w is not in the table
             if (!((ovf[bo[e] >> 9] >> ((bo[e] >> 4) & 31)) & 1u)) continue;
@@ -447,6 +450,12 @@ struct __attribute__((packed, aligned(4))) Ids4 {
     uint32_t x, y, z, w;
 };
 __device__ __forceinline__ Ids4 load_ids4(const uint32_t* p) { return *reinterpret_cast<const Ids4*>(p); }
+
+// A T list of n ids is walked 64 << sh ids per step, 1 << sh per lane: two
+// ids per lane up to 128 ids (one step, half the probes of four per lane),
+// four beyond.  One id per lane for lists up to 64 was slower: a step's four
+// (or two) independent bucket reads per lane are what hide its latency.
+__device__ __forceinline__ uint32_t lane_shift(uint32_t n) { return n <= 2 * KT_WAVE ? 1u : 2u; }
 
 // Binary-search probe of the sorted S(x) in global memory (huge tails).
 __device__ __forceinline__ uint32_t probe_bsearch(const uint32_t* __restrict__ s, uint64_t ns, uint64_t* pv,
@@ -523,9 +532,11 @@ struct ProbeCtx {
 // Streams the T lists of up to 64 neighbours R[k0, k0+cnt) through the probe.
 // Lane j first loads u_j and the bounds of T(u_j) (one coalesced load plus two
 // gathers for the whole chunk); the wave then walks the non-empty lists in
-// order, four ids per lane per step (one 16-byte load from the list start,
-// which needs only 4-byte alignment), prefetching the next block -- of the
-// same list or the next one -- before probing the current block.  Positions
+// order, four ids per lane per step (two for lists of at most 128 ids; one
+// 16-byte load per lane either way, which needs only 4-byte alignment: a load
+// whose width followed the list would have to complete before the branches
+// join), prefetching the next block -- of the same list or the next one --
+// before probing the current block.  Positions
 // and list bounds are wave-uniform 32-bit values; hits are counted per lane
 // and summed across the wave once per list.  Returns the chunk's hits.
 template <bool HASH, bool PLAIN, bool LIST>
@@ -547,7 +558,6 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
     }
     if (pending == 0) return 0;
     const uint32_t* __restrict__ T = V.t_heads();
-    const uint32_t l4 = 4 * lane;
 
     int j = __ffsll((unsigned long long)pending) - 1;
     pending &= pending - 1;
@@ -556,19 +566,21 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
     // |T(u)| < 2^32: an out-list holds distinct 32-bit ids
     uint32_t len = (uint32_t)(readlane64(tej, j) - readlane64(tbj, j));
     wk.stream += len;
-    uint32_t pos = 0;  // block start within the list (uniform)
+    uint32_t pos = 0;               // block start within the list (uniform)
+    uint32_t sh = lane_shift(len);  // 1 << sh ids per lane (uniform)
     // Every lane loads (lanes past the list end re-read its start; their ids
     // are masked by `valid`): an unconditional load keeps the number of
     // loads in flight static, so the prefetch below is not waited for early.
-    // Addresses are a wave-uniform base plus a 32-bit lane offset.
-    Ids4 q = load_ids4(base + (l4 < len ? l4 : 0u));
+    // Addresses are a wave-uniform base plus a 32-bit lane offset; at most
+    // three ids past a list's end are read.
+    Ids4 q = load_ids4(base + ((lane << sh) < len ? (lane << sh) : 0u));
     uint32_t nh = 0;    // this lane's hits on the current list
     uint32_t uacc = 0;  // hits on this lane's own list (T(uj)), added at the list's end
     uint64_t total = 0;
     while (true) {
         // next block: same list, or the first block of the next non-empty list
         const uint32_t* nbase = base;
-        uint32_t npos = pos + 4 * KT_WAVE, nlen = len, nu = u;
+        uint32_t npos = pos + (KT_WAVE << sh), nlen = len, nu = u, nsh = sh;
         int nj = j;
         bool more = true, switch_u = false;
         if (npos >= len) {
@@ -581,25 +593,36 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
                 nbase = T + ntb;
                 nlen = (uint32_t)(readlane64(tej, nj) - ntb);
                 wk.stream += nlen;
+                nsh = lane_shift(nlen);
                 npos = 0;
             } else {
                 more = false;
             }
         }
-        const Ids4 qn = load_ids4(nbase + (npos + l4 < nlen ? npos + l4 : 0u));
-        // probe the current block: id e of this lane is list entry pos + 4*lane + e
-        const int lim = (int)(len - pos) - (int)l4;
-        const bool valid[4] = {0 < lim, 1 < lim, 2 < lim, 3 < lim};
+        const uint32_t noff = npos + (lane << nsh);
+        const Ids4 qn = load_ids4(nbase + (noff < nlen ? noff : 0u));
+        // probe the current block: id e < (1 << sh) of this lane is list
+        // entry pos + (lane << sh) + e
+        const int lim = (int)(len - pos) - (int)(lane << sh);
+        const bool valid[4] = {0 < lim, 1 < lim, 2 < lim, 3 < lim};  // ids 2, 3 only when sh == 2
         const uint32_t w[4] = {q.x, q.y, q.z, q.w};
         if (HASH) {
-            if constexpr (LIST)
-                nh += probe4(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, [&](int e) { P.em(u, w[e]); });
-            else
-                nh += probe4(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, NoHitAction{});
+            if constexpr (LIST) {
+                auto hit = [&](int e) { P.em(u, w[e]); };
+                if (sh == 2)
+                    nh += probe_ids<4>(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, hit);
+                else
+                    nh += probe_ids<2>(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, hit);
+            } else {
+                if (sh == 2)
+                    nh += probe_ids<4>(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, NoHitAction{});
+                else
+                    nh += probe_ids<2>(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, NoHitAction{});
+            }
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                if (!valid[e]) continue;
+                if (e >= (1 << sh) || !valid[e]) continue;
                 const uint32_t h = probe_bsearch(P.s, P.ns, P.pv, w[e]);
                 nh += h;
                 if (LIST && h) P.em(u, w[e]);
@@ -617,6 +640,7 @@ __device__ __forceinline__ uint64_t scan_chunk(const View<PLAIN>& V, uint64_t k0
         base = nbase;
         len = nlen;
         pos = npos;
+        sh = nsh;
         q = qn;
     }
     // one atomic per neighbour with hits, issued once for the chunk
@@ -698,7 +722,8 @@ __device__ __forceinline__ uint64_t scan_chunk_grouped(const View<PLAIN>& V, uin
         const int lim = (int)clen - (int)(pos + s4);
         const bool valid[4] = {0 < lim, 1 < lim, 2 < lim, 3 < lim};
         const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
-        nh += probe4(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid, [&](int) { atomicAdd(&pk.ucnt[csrc], 1u); });
+        nh += probe_ids<4>(P.keys, P.cnt, P.ovf, P.boff, P.full, w, valid,
+                           [&](int) { atomicAdd(&pk.ucnt[csrc], 1u); });
         // advance; a group that switched lists fetches its list after next
         uint32_t fsrc, flen;
         const uint32_t* fbase;
