@@ -77,7 +77,8 @@ struct TcParams {
     uint32_t grid_per_cu;     // persistent workgroups per CU
     uint32_t strategy;        // TC_STREAMED or TC_GROUPED* (hash-probe tails)
     uint32_t order;           // TC_BY_DEGREE or TC_BY_VERTEX (work-queue order)
-    uint32_t table_shift = 4; // hash tables hold >= 2^table_shift slots per key where they fit
+    uint32_t table_shift = 3; // hash tables hold >= 2^table_shift slots per key where they fit (8: smaller
+                              // tables to clear and flush per tail; 16 gave the same RMAT-22 time, batches 1 % slower)
     uint32_t wave_grab = 32;  // wave-bin tails a wave takes per queue access (1..64), vertex order
     uint32_t bsearch_min_deg = 8192;  // |S(x)| above this: binary-search probes in global memory (<= 8192)
     uint32_t pseudo2core = 0; // skip tails with |S(x)| < 2 and count them (Config::pseudo2core)
