@@ -83,7 +83,7 @@ typedef struct {
 /* Intersection strategy knobs (bin thresholds and list walk); the package's
  * autotuner (katric_amd.tune) chooses them per graph class.  Counts never
  * depend on them. */
-#define KT_STRATEGY_STREAMED 0u  /* the wave walks one T list at a time, 256 ids per step (long lists) */
+#define KT_STRATEGY_STREAMED 0u  /* the wave walks one T list at a time, 256 ids per step (128 for lists of <= 128) */
 #define KT_STRATEGY_GROUPED16 1u /* groups of 16 lanes walk 4 T lists at a time (short lists) */
 #define KT_STRATEGY_GROUPED8 2u  /* groups of 8 lanes walk 8 T lists at a time (very short lists) */
 #define KT_STRATEGY_GROUPED32 3u /* groups of 32 lanes walk 2 T lists at a time */
