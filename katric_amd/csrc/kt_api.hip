@@ -537,13 +537,15 @@ int kt::count_locked(kt_ctx* c, uint32_t n, const uint64_t* offsets, const uint3
     }
     TcOut out{pv, totals};
     trace("intersection: X=%u pv=%p", n, (void*)pv);
-    // one graph's wave tails vary in work: smaller grabs even out the end of
-    // the launch (RMAT-22: k_tc_main 21.0 ms with 32 tails per grab, 20.6
-    // with 16, 20.4-20.6 with 8; G(n,m) 2^16: 0.205 ms with 16, 0.187 with 8;
-    // the shard and batch views, whose tails are many and tiny, are fastest
-    // with 32)
+    // wave-bin tails per queue access for one graph: each grab costs a queue
+    // atomic and one round trip for the tails' bounds, and a larger grab
+    // leaves more uneven work at the end of the launch.  With the min-side
+    // walk: RMAT-22 k_tc_main 17.0 ms with 4 tails per grab, 13.9 with 8, 12.6
+    // with 16 or 24, 12.6 with 32, 12.9 with 64; G(n,m) 2^16 0.207 ms with 8,
+    // 0.170 with 16, 0.218 with 32.  (The forward walk was fastest with 8.)
+    // The shard and batch views, whose tails are many and tiny, take 32.
     TcParams pp = c->prm;
-    pp.wave_grab = 8;
+    pp.wave_grab = 16;
     rc = c->hip_check(tc_launch(view, out, pp, B.ws_tc, c->num_cus, st, c->ev[EV_TC0], c->ev[EV_TC1]),
                       "intersection launch");
     if (rc) return rc;
