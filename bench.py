@@ -43,11 +43,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # per dispatch).  A plain 1 GiB device copy reads FETCH_SIZE = 524,308 KiB and
 # WRITE_SIZE = 1,048,576 KiB on MI355X, so FETCH_SIZE is scaled by 2.000 and
 # WRITE_SIZE by 1.000.  RMAT-22 ef16 seed 1 (min-side walk):
-# 34,998,888 KiB x 2.000 + 3,506,352 KiB = 75.27 GB.
+# 34,703,157 KiB x 2.000 + 3,177,102 KiB = 74.33 GB.
 # Shard (RMAT-26 ef16 seed 1, 16 ranks, rank 0, CeTriC), the count phase's
 # intersection launches (k_tc_main + k_tc_big, local and contracted, binning
 # and k_tc_huge): FETCH_SIZE 292,888,269 KiB x 2.000 + WRITE_SIZE 15,960,809 KiB = 616.18 GB.
-MEASURED_TRAFFIC = {("plain", 22, 16, 1): 75.27e9, ("shard-cetric", 26, 16, 1, 16, 0): 616.18e9}
+MEASURED_TRAFFIC = {("plain", 22, 16, 1): 74.33e9, ("shard-cetric", 26, 16, 1, 16, 0): 616.18e9}
 
 
 
